@@ -1,0 +1,156 @@
+/*
+ * nbody.h — C ABI of the MI355X (gfx950) N-body force engine.
+ *
+ * One opaque context (nb_ctx) owns every device buffer of one particle
+ * system: FP64 positions/velocities/accelerations, the FP32 (x,y,z,m)
+ * force view, the octree and all scratch. State stays resident in HBM
+ * between calls; host memory is only touched by nb_upload / nb_download*.
+ *
+ * Each entry point replaces a piece of the reference's GPU solver plumbing:
+ *   ParticleSolver::updateParticlePositions  include/ParticleSolver.h:10
+ *   ParticleSolverGPU (v4 tiled direct sum)   src/ParticleSolver/ParticleSolverGPU/ParticleSolverGPU.cpp:18-41
+ *   ParticleSolverBHutGPU (v7 Barnes-Hut)     src/ParticleSolver/ParticleSolverBHutGPU/ParticleSolverBHutGPU.cpp:9-61
+ *   ParticleSystem SSBO set-up (bindings 0-4) src/ParticleSystem/ParticleSystem.cpp:73-83
+ *   OpenGLBuffer create/map/read-back        src/OpenGLBuffer/OpenGLBuffer.cpp:12-165
+ *
+ * Conventions (reference semantics):
+ *   - "force" is the acceleration a_i = G sum_{j!=i} m_j r_ij / (|r_ij|^2 + eps2)^{3/2}
+ *     with no m_i factor (ParticleSolverCPUSequential.cpp:26-45, ParticleSystem.h:43-46).
+ *   - Host arrays use the reference's per-attribute vec4 layout: pos4/vel4/acc4
+ *     are float[4*n] with w ignored on input and written as 0 on output;
+ *     masses are a plain float[n] (the reference keeps them in vec4.x).
+ *   - theta is given unsquared (0.5); the opening test is s^2 < theta^2 d^2
+ *     (Octree.cpp:15,320; ParticleSolverBHutGPU.h:19).
+ *
+ * Threading: a context is used by one host thread at a time; distinct
+ * contexts may be used concurrently. nb_step is asynchronous on the
+ * context's stream; nb_download*, nb_forces, nb_energy and nb_sync wait.
+ * Every function returns 0 on success or a negative NB_E_* code; the
+ * calling thread's nb_last_error() then describes the failure.
+ */
+#ifndef NBODY_H
+#define NBODY_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+#define NB_OK 0
+#define NB_E_ARG (-1)           /* bad argument / size / state */
+#define NB_E_HIP (-2)           /* HIP runtime error */
+#define NB_E_OOM (-3)           /* device allocation failed */
+#define NB_E_BUDGET (-4)        /* request exceeds the HBM admission budget */
+#define NB_E_TREE_OVERFLOW (-5) /* octree node pool exhausted */
+#define NB_E_NONFINITE (-6)     /* a NaN/Inf appeared in the particle state */
+#define NB_E_NOGPU (-7)         /* no HIP device visible */
+
+/* ---- configuration ---------------------------------------------------- */
+typedef enum nb_solver {
+  NB_SOLVER_DIRECT = 0, /* O(N^2) tiled particle-particle (replaces v4) */
+  NB_SOLVER_BH = 1      /* Barnes-Hut octree (replaces v7) */
+} nb_solver;
+
+typedef enum nb_integrator {
+  /* CPU solvers' lagged kick-drift-kick: v+=a*dt/2; x+=v*dt; a=F(x_pre); v+=a*dt/2
+   * (ParticleSystem.h:34-51). Default. */
+  NB_INTEG_KDK_LAGGED = 0,
+  /* GPU solvers' semi-implicit Euler: v+=F*dt; x+=v*dt (updateParticles.glsl:44-55). */
+  NB_INTEG_EULER = 1
+} nb_integrator;
+
+typedef struct nb_config {
+  int64_t n;          /* number of bodies, 1 <= n < 2^30 */
+  float eps2;         /* squared Plummer softening (reference "-s") */
+  float dt;           /* time step (reference "-t") */
+  float G;            /* gravitational constant; 0 means 1.0 (reference G=1) */
+  float theta;        /* BH opening angle, unsquared; 0 means 0.5 */
+  int32_t solver;     /* nb_solver */
+  int32_t integrator; /* nb_integrator */
+  int32_t leaf_size;  /* BH bodies per leaf, 1..64; 0 = tuned default */
+  int32_t bodies_per_lane; /* direct: i-bodies per lane (1,2,4); 0 = tuned default */
+  int32_t j_split;    /* direct: waves sharing one i-block (1,2,4); 0 = tuned default */
+  int32_t flags;      /* NB_F_* */
+} nb_config;
+
+#define NB_F_NO_GRAPH 0x1 /* do not capture multi-step nb_step calls into a hipGraph */
+
+typedef struct nb_ctx nb_ctx;
+
+/* Create a context and allocate every device buffer for cfg->n bodies. */
+int nb_create(nb_ctx** out, const nb_config* cfg);
+void nb_destroy(nb_ctx* ctx);
+const char* nb_last_error(void);
+
+/* Upload the particle system (replaces ParticleSystem.cpp:75-79 glBufferData at
+ * bindings 0-3). acc4 may be NULL (zero initial acceleration, Particle.cpp:4). */
+int nb_upload(nb_ctx* ctx, const float* pos4, const float* vel4, const float* acc4,
+              const float* mass, int64_t n);
+/* FP64 variant: pos3/vel3/acc3 are double[3*n] (x,y,z interleaved). */
+int nb_upload_f64(nb_ctx* ctx, const double* pos3, const double* vel3, const double* acc3,
+                  const float* mass, int64_t n);
+
+/* Advance nsteps full steps (forces + integration), asynchronously
+ * (replaces ParticleSolver::updateParticlePositions, ParticleSolver.h:10). */
+int nb_step(nb_ctx* ctx, int32_t nsteps);
+int nb_sync(nb_ctx* ctx);
+
+/* Force-only pass at the current positions, no integration; writes float[4*n]
+ * (fx,fy,fz,0) — the reference's forces buffer, binding 4. */
+int nb_forces(nb_ctx* ctx, float* force4_out);
+/* FP64 view of the same forces (compensated FP32 sum widened): double[3*n]. */
+int nb_forces_f64(nb_ctx* ctx, double* force3_out);
+
+int nb_download(nb_ctx* ctx, float* pos4, float* vel4, float* acc4);
+int nb_download_f64(nb_ctx* ctx, double* pos3, double* vel3, double* acc3);
+
+/* FP64 diagnostics: kinetic and softened potential energy, linear momentum.
+ * epot = -G sum_{i<j} m_i m_j / sqrt(r^2 + eps2). */
+int nb_energy(nb_ctx* ctx, double* ekin, double* epot, double p[3]);
+
+/* ---- Barnes-Hut debug taps (valid after a BH nb_step or nb_forces) ---- */
+/* codes[n]: 30-bit Morton code of body i (unsorted order);
+ * perm[n]: body index at sorted position k (sorted by (code, index)). */
+int nb_debug_morton(nb_ctx* ctx, uint32_t* codes, uint32_t* perm);
+/* Flattened octree, breadth-first numbering, children in Morton-digit order. */
+typedef struct nb_node_view {
+  float com[3];         /* centre of mass */
+  float mass;
+  int32_t first_child;  /* -1 for a leaf */
+  int32_t next;         /* rope: next node in pre-order after this subtree, -1 at end */
+  int32_t body_start;   /* first sorted body of the node's range */
+  int32_t body_count;   /* bodies in the range */
+  int32_t level;        /* 0 = root; cell side = root side / 2^level */
+  int32_t pad_;
+} nb_node_view;
+/* Writes min(cap, node_count) nodes; *node_count receives the total. The
+ * root bounding box is written to bbox[6] = (min xyz, max xyz) if non-NULL. */
+int nb_debug_tree(nb_ctx* ctx, nb_node_view* nodes, int64_t cap, int64_t* node_count,
+                  float* bbox);
+
+/* ---- timing ------------------------------------------------------------ */
+typedef struct nb_timing {
+  double force_ms;      /* summed device time of force kernels (direct or traversal) */
+  int64_t force_calls;
+  double step_ms;       /* summed device time of whole steps (first to last kernel) */
+  int64_t step_calls;
+  double build_ms;      /* BH: bbox + encode + sort + build + COM */
+  int64_t build_calls;
+  int64_t interactions; /* BH: counted pair interactions of the last counted pass */
+} nb_timing;
+/* Enable (1) or disable (0) HIP-event timing of each step's phases; resets totals. */
+int nb_set_timing(nb_ctx* ctx, int32_t enable);
+int nb_get_timing(nb_ctx* ctx, nb_timing* out);
+/* BH: count body-cell + body-body interactions at the current state (diagnostic
+ * traversal, not on the step path). */
+int nb_count_interactions(nb_ctx* ctx, int64_t* interactions);
+
+/* Library / device description for logs: writes a NUL-terminated string. */
+int nb_device_info(char* buf, int32_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NBODY_H */

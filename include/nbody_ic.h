@@ -1,0 +1,57 @@
+/*
+ * nbody_ic.h — seeded initial conditions (host-only, part of libnbody).
+ *
+ * Counterpart of the reference's ParticleSystemInitializer::generateParticles
+ * (include/ParticleSystemInitializer.h; src/ParticleSystemInitializer/*), whose
+ * random initializers are unseeded (ParticleSystemBall.cpp:15-16,
+ * ParticleSystemGalaxyInitializer.cpp:17-18). Here every generator is a pure
+ * function of (seed, parameters): a SplitMix64 stream gives uniform doubles in
+ * (0, 1), so CPU checks and GPU runs start from identical bytes.
+ *
+ * Outputs: pos3/vel3 are double[3n] (x, y, z interleaved), mass is float[n].
+ * All return 0 on success or NB_E_ARG (-1).
+ */
+#ifndef NBODY_IC_H
+#define NBODY_IC_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Uniform ball: r = radius * u^(1/3), isotropic direction, zero velocity. */
+int nb_ic_ball(uint64_t seed, int64_t n, double radius, double mass_each, double* pos3,
+               double* vel3, float* mass);
+
+/* Plummer sphere (Aarseth, Henon & Wielen 1974): scale radius a, total mass M,
+ * radii a / sqrt(u^(-2/3) - 1) resampled above rmax_over_a * a, speeds q * v_esc(r)
+ * with q drawn by rejection from q^2 (1 - q^2)^(7/2), isotropic directions; the
+ * centre of mass and its velocity are moved to the origin. */
+int nb_ic_plummer(uint64_t seed, int64_t n, double a, double total_mass, double rmax_over_a,
+                  double G, double* pos3, double* vel3, float* mass);
+
+/* Two exponential discs of n/2 bodies each (n even). Per disc: body 0 is a central
+ * point mass of central_factor * m_b; the others have R ~ Gamma(2, scale_length)
+ * (i.e. -scale_length * ln(u1 u2)), z = scale_height * atanh(2u - 1) (sech^2
+ * layer), circular speed sqrt(G M(<R) R^2 / (R^2 + eps2)^(3/2)) with M(<R) the
+ * central mass plus the enclosed exponential-disc mass. m_b = total_mass / n.
+ * Disc centres at x = -separation/2 and +separation/2 with bulk velocities
+ * +closing_speed/2 and -closing_speed/2 along x. */
+int nb_ic_discs(uint64_t seed, int64_t n, double scale_length, double scale_height,
+                double central_factor, double total_mass, double separation,
+                double closing_speed, double eps2, double G, double* pos3, double* vel3,
+                float* mass);
+
+/* The reference's deterministic Lagrange triangle, 3 bodies
+ * (ParticleSystemLagrange.cpp:8-50), computed in single precision as there. */
+int nb_ic_lagrange(double world_size, double* pos3, double* vel3, float* mass);
+
+/* Log-uniform integers floor(exp(U(ln lo, ln hi))) in [lo, hi]. */
+int nb_ic_log_uniform_sizes(uint64_t seed, int32_t count, int32_t lo, int32_t hi, int32_t* out);
+/* Uniform doubles in [lo, hi). */
+int nb_ic_uniform(uint64_t seed, int32_t count, double lo, double hi, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,826 @@
+// nb_bh.hip — Barnes-Hut force pass on gfx950.
+//
+// Replaces the reference's GPU octree (ParallelOctreeGPU.cpp:77-300 with the
+// shaders boundingBoxPass1/2, resetOctree, expandOctree, countParticlesPerTask,
+// prefixSum, addBlockSums, distributeParticles, insertParticles,
+// propagateFatherOctree, pruneFather, pruneSubtrees) and the traversal shader
+// forceCalcuBarnesHut.glsl:54-118, driven by ParticleSolverBHutGPU.cpp:25-50.
+// It uses the reference's dormant Morton encoding (morton.glsl:37-98) as the
+// spatial key instead of top-down bucketing.
+//
+// Pipeline per force pass (all on one stream, no host synchronisation, so the
+// whole step can be captured into a hipGraph):
+//   1. bbox        per-block min/max, then one block folds the partials and
+//                  writes the root box and the cell side^2 of each level.
+//   2. encode      30-bit Morton code per body (bit-exact restatement of
+//                  morton.glsl: IEEE divide, clamp, 10-bit quantisation).
+//   3. sort        LSD radix sort of (code, body index), 4 passes of 8 bits.
+//                  Each pass: block digit histograms, a device-wide exclusive
+//                  scan, and a stable scatter whose per-wave ranks come from
+//                  64-bit ballots (8 ballots give each lane the mask of lanes
+//                  holding the same digit). Stable => sorted by (code, index).
+//   4. permute     gather (x, y, z, m) into sorted order.
+//   5. prefix      FP64 exclusive prefix of (m, m x, m y, m z) over sorted bodies:
+//                  any node's mass and centre of mass is a difference of two
+//                  prefix entries (cells are contiguous code ranges).
+//   6. octree      breadth-first, one level per round: a node with more than
+//                  leaf_size bodies (and level < 10) splits into its non-empty
+//                  Morton-digit children, found by binary search in its code
+//                  range; children get consecutive indices from a scan of child
+//                  counts, so numbering is deterministic. Ropes (`next`) point to
+//                  the next sibling, or the parent's rope for the last child —
+//                  the firstChild/next contract of Octree.cpp:264-294.
+//   7. com         per node from the prefix; single-body nodes copy the body.
+//   8. traverse    one lane per sorted body, stackless rope walk with the
+//                  reference's opening test s^2 < theta^2 d^2 (Octree.cpp:320);
+//                  a leaf that is not accepted is evaluated body by body.
+#include <algorithm>
+#include <cfloat>
+#include <vector>
+
+#include "nb_internal.h"
+
+namespace {
+
+constexpr int kBboxBlocks = 1024;
+constexpr int kSortThreads = 256;           // 4 waves
+constexpr int kSortItems = 16;              // elements per lane per pass
+constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 elements per block
+constexpr int kScanTile = 2048;             // elements per block in the scans
+constexpr int kScanThreads = 256;
+
+// ---- 1. bounding box --------------------------------------------------------
+__global__ __launch_bounds__(256) void k_bbox_partial(const float4* __restrict__ pm, int n,
+                                                      float* __restrict__ part) {
+  float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+  float mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const float4 p = pm[i];
+    mn[0] = fminf(mn[0], p.x); mx[0] = fmaxf(mx[0], p.x);
+    mn[1] = fminf(mn[1], p.y); mx[1] = fmaxf(mx[1], p.y);
+    mn[2] = fminf(mn[2], p.z); mx[2] = fmaxf(mx[2], p.z);
+  }
+  __shared__ float red[6][256];
+  for (int a = 0; a < 3; ++a) {
+    red[a][threadIdx.x] = mn[a];
+    red[3 + a][threadIdx.x] = mx[a];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int a = 0; a < 3; ++a) {
+        red[a][threadIdx.x] = fminf(red[a][threadIdx.x], red[a][threadIdx.x + s]);
+        red[3 + a][threadIdx.x] = fmaxf(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + s]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[6 * blockIdx.x + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ __launch_bounds__(256) void k_bbox_final(const float* __restrict__ part, int nparts,
+                                                    float* __restrict__ geom) {
+  float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+  float mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (int b = threadIdx.x; b < nparts; b += 256)
+    for (int a = 0; a < 3; ++a) {
+      mn[a] = fminf(mn[a], part[6 * b + a]);
+      mx[a] = fmaxf(mx[a], part[6 * b + 3 + a]);
+    }
+  __shared__ float red[6][256];
+  for (int a = 0; a < 3; ++a) {
+    red[a][threadIdx.x] = mn[a];
+    red[3 + a][threadIdx.x] = mx[a];
+  }
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s)
+      for (int a = 0; a < 3; ++a) {
+        red[a][threadIdx.x] = fminf(red[a][threadIdx.x], red[a][threadIdx.x + s]);
+        red[3 + a][threadIdx.x] = fmaxf(red[3 + a][threadIdx.x], red[3 + a][threadIdx.x + s]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    float side = 0.f;
+    for (int a = 0; a < 3; ++a) {
+      geom[a] = red[a][0];
+      geom[3 + a] = red[3 + a][0];
+      const float ext = fmaxf(__fsub_rn(red[3 + a][0], red[a][0]), 1e-6f);
+      side = fmaxf(side, ext);
+    }
+    for (int l = 0; l <= kMaxLevel; ++l) {
+      const float s = ldexpf(side, -l);
+      geom[6 + l] = __fmul_rn(s, s);
+    }
+  }
+}
+
+// ---- 2. Morton encode (morton.glsl:37-98) ------------------------------------
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
+  v = (v * 0x00010001u) & 0xFF0000FFu;
+  v = (v * 0x00000101u) & 0x0F00F00Fu;
+  v = (v * 0x00000011u) & 0xC30C30C3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__device__ __forceinline__ uint32_t quantize(float p, float lo, float hi) {
+  const float ext = fmaxf(__fsub_rn(hi, lo), 1e-6f);
+  float nrm = __fdiv_rn(__fsub_rn(p, lo), ext);
+  nrm = fminf(fmaxf(nrm, 0.f), 1.f);
+  const float g = fminf(fmaxf(__fmul_rn(nrm, 1024.f), 0.f), 1023.f);
+  return (uint32_t)g;
+}
+
+__global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ pm, int n,
+                                                const float* __restrict__ geom,
+                                                uint32_t* __restrict__ codes,
+                                                uint32_t* __restrict__ keys,
+                                                uint32_t* __restrict__ vals) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float4 p = pm[i];
+  const uint32_t qx = quantize(p.x, geom[0], geom[3]);
+  const uint32_t qy = quantize(p.y, geom[1], geom[4]);
+  const uint32_t qz = quantize(p.z, geom[2], geom[5]);
+  const uint32_t code = expand_bits(qx) * 4u + expand_bits(qy) * 2u + expand_bits(qz);
+  codes[i] = code;
+  keys[i] = code;
+  vals[i] = (uint32_t)i;
+}
+
+// ---- generic reduce-then-scan (int32 and FP64x4) ----------------------------
+// The element count is either a constant or the size of one octree level,
+// read on the device: count = level_off[l + 1] - level_off[l].
+// The count is clamped to the launch bound `cap`, so a bad level table can
+// never send the scan past its buffers.
+__device__ __forceinline__ int scan_count(const int* level_off, int l, int n_const, int cap) {
+  const int c = level_off ? level_off[l + 1] - level_off[l] : n_const;
+  return c < 0 ? 0 : (c > cap ? cap : c);
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_iscan_reduce(const int* __restrict__ in,
+                                                               const int* level_off, int l,
+                                                               int n_const, int cap,
+                                                               int* __restrict__ part) {
+  const int n = scan_count(level_off, l, n_const, cap);
+  const int base = blockIdx.x * kScanTile;
+  int s = 0;
+  for (int k = threadIdx.x; k < kScanTile; k += kScanThreads) {
+    const int e = base + k;
+    if (e < n) s += in[e];
+  }
+  __shared__ int red[kScanThreads];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kScanThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Exclusive scan of nparts block sums by one block; writes the grand total to
+// part[nparts]. When level_off is given it also publishes the next level's end:
+// level_off[l + 2] = level_off[l + 1] + total.
+__global__ __launch_bounds__(kScanThreads) void k_iscan_partials(int* __restrict__ part,
+                                                                 int nparts, int* level_off,
+                                                                 int l) {
+  __shared__ int sh[kScanThreads];
+  int carry = 0;
+  for (int b0 = 0; b0 < nparts; b0 += kScanThreads) {
+    const int b = b0 + threadIdx.x;
+    const int v = b < nparts ? part[b] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < kScanThreads; d <<= 1) {
+      const int t = threadIdx.x >= d ? sh[threadIdx.x - d] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += t;
+      __syncthreads();
+    }
+    if (b < nparts) part[b] = carry + sh[threadIdx.x] - v;
+    carry += sh[kScanThreads - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[nparts] = carry;
+    if (level_off) level_off[l + 2] = level_off[l + 1] + carry;
+  }
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_iscan_final(const int* __restrict__ in,
+                                                              const int* level_off, int l,
+                                                              int n_const, int cap,
+                                                              const int* __restrict__ part,
+                                                              int* __restrict__ out) {
+  const int n = scan_count(level_off, l, n_const, cap);
+  const int base = blockIdx.x * kScanTile;
+  if (base >= n) return;
+  constexpr int kPer = kScanTile / kScanThreads;  // 8 consecutive elements per thread
+  int v[kPer];
+  int s = 0;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int e = base + threadIdx.x * kPer + k;
+    v[k] = e < n ? in[e] : 0;
+    s += v[k];
+  }
+  __shared__ int sh[kScanThreads];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = 1; d < kScanThreads; d <<= 1) {
+    const int t = threadIdx.x >= d ? sh[threadIdx.x - d] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int run = part[blockIdx.x] + sh[threadIdx.x] - s;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int e = base + threadIdx.x * kPer + k;
+    if (e < n) out[e] = run;
+    run += v[k];
+  }
+}
+
+// FP64 prefix of (m, m x, m y, m z): prefix[k] = sum over sorted bodies < k.
+__device__ __forceinline__ double4 body_moments(float4 p) {
+  const double m = p.w;
+  return make_double4(m, m * (double)p.x, m * (double)p.y, m * (double)p.z);
+}
+__device__ __forceinline__ double4 add4(double4 a, double4 b) {
+  return make_double4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_mscan_reduce(const float4* __restrict__ pms,
+                                                               int n, double4* __restrict__ part) {
+  constexpr int kPer = kScanTile / kScanThreads;
+  const int base = blockIdx.x * kScanTile;
+  double4 s = make_double4(0, 0, 0, 0);
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int e = base + threadIdx.x * kPer + k;
+    if (e < n) s = add4(s, body_moments(pms[e]));
+  }
+  __shared__ double4 sh[kScanThreads];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = kScanThreads / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] = add4(sh[threadIdx.x], sh[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+// Exclusive scan of the block sums by one block, in a fixed order: each thread
+// folds a contiguous run of partials, a Hillis-Steele pass scans the 256 run
+// totals, then each thread rewrites its run. part[nparts] receives the total.
+__global__ __launch_bounds__(kScanThreads) void k_mscan_partials(double4* __restrict__ part,
+                                                                 int nparts) {
+  const int per = (nparts + kScanThreads - 1) / kScanThreads;
+  const int b0 = threadIdx.x * per;
+  const int b1 = min(b0 + per, nparts);
+  double4 tot = make_double4(0, 0, 0, 0);
+  for (int b = b0; b < b1; ++b) tot = add4(tot, part[b]);
+  __shared__ double4 sh[kScanThreads];
+  sh[threadIdx.x] = tot;
+  __syncthreads();
+  for (int d = 1; d < kScanThreads; d <<= 1) {
+    const double4 t = threadIdx.x >= d ? sh[threadIdx.x - d] : make_double4(0, 0, 0, 0);
+    __syncthreads();
+    sh[threadIdx.x] = add4(sh[threadIdx.x], t);
+    __syncthreads();
+  }
+  double4 run = threadIdx.x > 0 ? sh[threadIdx.x - 1] : make_double4(0, 0, 0, 0);
+  for (int b = b0; b < b1; ++b) {
+    const double4 v = part[b];
+    part[b] = run;
+    run = add4(run, v);
+  }
+  if (threadIdx.x == kScanThreads - 1) part[nparts] = sh[kScanThreads - 1];
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_mscan_final(const float4* __restrict__ pms,
+                                                              int n,
+                                                              const double4* __restrict__ part,
+                                                              double4* __restrict__ prefix) {
+  constexpr int kPer = kScanTile / kScanThreads;
+  const int base = blockIdx.x * kScanTile;
+  double4 v[kPer];
+  double4 s = make_double4(0, 0, 0, 0);
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int e = base + threadIdx.x * kPer + k;
+    v[k] = e < n ? body_moments(pms[e]) : make_double4(0, 0, 0, 0);
+    s = add4(s, v[k]);
+  }
+  __shared__ double4 sh[kScanThreads];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int d = 1; d < kScanThreads; d <<= 1) {
+    const double4 t = threadIdx.x >= d ? sh[threadIdx.x - d] : make_double4(0, 0, 0, 0);
+    __syncthreads();
+    sh[threadIdx.x] = add4(sh[threadIdx.x], t);
+    __syncthreads();
+  }
+  // exclusive start of this thread: block offset + totals of the lower threads
+  double4 run = threadIdx.x > 0 ? add4(part[blockIdx.x], sh[threadIdx.x - 1]) : part[blockIdx.x];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    const int e = base + threadIdx.x * kPer + k;
+    if (e < n) prefix[e] = run;
+    run = add4(run, v[k]);
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) prefix[n] = part[gridDim.x];
+}
+
+// ---- 3. radix sort ------------------------------------------------------------
+__global__ __launch_bounds__(kSortThreads) void k_radix_hist(const uint32_t* __restrict__ keys,
+                                                             int n, int shift, int nblocks,
+                                                             int* __restrict__ hist) {
+  __shared__ int h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * kSortTile;
+#pragma unroll 4
+  for (int r = 0; r < kSortItems; ++r) {
+    const int e = base + r * kSortThreads + threadIdx.x;
+    if (e < n) atomicAdd(&h[(keys[e] >> shift) & 255u], 1);
+  }
+  __syncthreads();
+  hist[threadIdx.x * nblocks + blockIdx.x] = h[threadIdx.x];
+}
+
+// Mask of the active lanes whose 8-bit digit equals this lane's digit.
+__device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
+  uint64_t m = active;
+#pragma unroll
+  for (int b = 0; b < 8; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const uint64_t bb = __ballot(bit);
+    m &= bit ? bb : ~bb;
+  }
+  return m;
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
+    const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, int n, int shift,
+    int nblocks, const int* __restrict__ offsets, uint32_t* __restrict__ keys_out,
+    uint32_t* __restrict__ vals_out) {
+  __shared__ int whist[4][256];
+  __shared__ int wrun[4][256];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  for (int w = 0; w < 4; ++w) whist[w][threadIdx.x] = 0;
+  __syncthreads();
+
+  // Wave w owns tile elements [w*1024, (w+1)*1024), 64 consecutive per round.
+  const int base = blockIdx.x * kSortTile + wave * (kSortTile / 4);
+  const uint64_t lt = (uint64_t(1) << lane) - 1;
+  uint32_t k[kSortItems], v[kSortItems];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int e = base + r * 64 + lane;
+    k[r] = e < n ? keys_in[e] : 0u;
+    v[r] = e < n ? vals_in[e] : 0u;
+  }
+  // phase 1: per-wave digit counts
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int e = base + r * 64 + lane;
+    const uint64_t active = __ballot(e < n);
+    const uint32_t d = (k[r] >> shift) & 255u;
+    const uint64_t m = match_digit(d, active);
+    if (e < n && (m & lt) == 0) whist[wave][d] += __popcll(m);
+  }
+  __syncthreads();
+  {
+    const int d = threadIdx.x;
+    int run = offsets[d * nblocks + blockIdx.x];
+    for (int w = 0; w < 4; ++w) {
+      wrun[w][d] = run;
+      run += whist[w][d];
+    }
+  }
+  __syncthreads();
+  // phase 2: stable ranks and scatter
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const int e = base + r * 64 + lane;
+    const uint64_t active = __ballot(e < n);
+    const uint32_t d = (k[r] >> shift) & 255u;
+    const uint64_t m = match_digit(d, active);
+    if (e < n) {
+      const int dst = wrun[wave][d] + __popcll(m & lt);
+      if (dst >= 0 && dst < n) {  // always true for a consistent histogram
+        keys_out[dst] = k[r];
+        vals_out[dst] = v[r];
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (e < n && (m & lt) == 0) wrun[wave][d] += __popcll(m);
+    __builtin_amdgcn_wave_barrier();
+  }
+}
+
+// ---- 4. permute -------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_gather(const float4* __restrict__ pm,
+                                                const uint32_t* __restrict__ perm, int n,
+                                                float4* __restrict__ pms) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    const uint32_t p = perm[i];
+    pms[i] = p < (uint32_t)n ? pm[p] : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// ---- 6. octree ------------------------------------------------------------------------
+__device__ __forceinline__ int link_count(int w) { return w >> 4; }
+__device__ __forceinline__ int link_level(int w) { return w & 15; }
+
+__global__ void k_tree_init(int n, int4* __restrict__ link, int* __restrict__ level_off) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  link[0] = make_int4(-1, -1, 0, n << 4);
+  level_off[0] = 0;
+  level_off[1] = 1;
+  for (int l = 2; l <= kMaxLevel + 1; ++l) level_off[l] = 1;
+}
+
+// First index in [lo, hi) whose digit (codes >> shift) & 7 is >= d (digits are sorted).
+__device__ __forceinline__ int digit_lower_bound(const uint32_t* __restrict__ codes, int lo,
+                                                 int hi, int shift, uint32_t d) {
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (((codes[mid] >> shift) & 7u) < d) lo = mid + 1;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void k_tree_count(const uint32_t* __restrict__ codes, int n,
+                                                    const int4* __restrict__ link,
+                                                    const int* __restrict__ level_off, int l,
+                                                    int leaf, int cap,
+                                                    int* __restrict__ child_cnt) {
+  const int lo = level_off[l];
+  const int hi = min(level_off[l + 1], lo + cap);  // cap = launch bound of this level
+  const int shift = 27 - 3 * l;
+  for (int k = lo + blockIdx.x * 256 + threadIdx.x; k < hi; k += gridDim.x * 256) {
+    const int4 nd = link[k];
+    const int cnt = link_count(nd.w);
+    int nc = 0;
+    if (cnt > leaf && nd.z >= 0 && nd.z + cnt <= n) {
+      const int s = nd.z, e = nd.z + cnt;
+      int b = s;
+      for (uint32_t d = 1; d <= 8; ++d) {
+        const int c = d < 8 ? digit_lower_bound(codes, b, e, shift, d) : e;
+        nc += c > b;
+        b = c;
+      }
+    }
+    child_cnt[k - lo] = nc;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_tree_emit(const uint32_t* __restrict__ codes, int n,
+                                                   int4* __restrict__ link,
+                                                   const int* __restrict__ level_off, int l,
+                                                   int leaf, int cap,
+                                                   const int* __restrict__ child_off,
+                                                   int64_t max_nodes, int* __restrict__ flags) {
+  const int lo = level_off[l];
+  const int hi = min(level_off[l + 1], lo + cap);
+  const int next_lo = level_off[l + 1];
+  const int shift = 27 - 3 * l;
+  for (int k = lo + blockIdx.x * 256 + threadIdx.x; k < hi; k += gridDim.x * 256) {
+    int4 nd = link[k];
+    const int cnt = link_count(nd.w);
+    if (cnt <= leaf || nd.z < 0 || nd.z + cnt > n) continue;
+    const int first = next_lo + child_off[k - lo];
+    const int s = nd.z, e = nd.z + cnt;
+    int starts[8], ends[8], nc = 0;
+    int b = s;
+    for (uint32_t d = 1; d <= 8; ++d) {
+      const int c = d < 8 ? digit_lower_bound(codes, b, e, shift, d) : e;
+      if (c > b) {
+        starts[nc] = b;
+        ends[nc] = c;
+        ++nc;
+      }
+      b = c;
+    }
+    if ((int64_t)first + nc > max_nodes) {
+      atomicOr(flags, 2);
+      continue;
+    }
+    for (int q = 0; q < nc; ++q) {
+      const int idx = first + q;
+      const int nxt = q + 1 < nc ? idx + 1 : nd.y;
+      link[idx] = make_int4(-1, nxt, starts[q], ((ends[q] - starts[q]) << 4) | (l + 1));
+    }
+    link[k].x = first;
+  }
+}
+
+// ---- 7. centre of mass ------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_tree_com(const int4* __restrict__ link, int n,
+                                                  int64_t max_nodes,
+                                                  const int* __restrict__ level_off,
+                                                  const double4* __restrict__ prefix,
+                                                  const float4* __restrict__ pms,
+                                                  float4* __restrict__ com) {
+  const int total = (int)min((int64_t)level_off[kMaxLevel + 1], max_nodes);
+  for (int k = blockIdx.x * 256 + threadIdx.x; k < total; k += gridDim.x * 256) {
+    const int4 nd = link[k];
+    const int cnt = link_count(nd.w);
+    if (nd.z < 0 || cnt < 1 || nd.z + cnt > n) {
+      com[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+      continue;
+    }
+    if (cnt == 1) {
+      com[k] = pms[nd.z];
+      continue;
+    }
+    const double4 a = prefix[nd.z], b = prefix[nd.z + cnt];
+    const double m = b.x - a.x;
+    const double inv = m > 0.0 ? 1.0 / m : 0.0;
+    com[k] = make_float4((float)((b.y - a.y) * inv), (float)((b.z - a.z) * inv),
+                         (float)((b.w - a.w) * inv), (float)m);
+  }
+}
+
+// ---- 8. traversal ---------------------------------------------------------------------------
+template <bool kCount>
+__global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ pms,
+                                                  const uint32_t* __restrict__ perm, int n,
+                                                  const float4* __restrict__ com,
+                                                  const int4* __restrict__ link,
+                                                  const int* __restrict__ level_off,
+                                                  int64_t max_nodes,
+                                                  const float* __restrict__ geom, float theta2,
+                                                  float eps2, double G,
+                                                  double* __restrict__ force,
+                                                  unsigned long long* __restrict__ inter,
+                                                  int* __restrict__ flags) {
+  __shared__ float side2[kMaxLevel + 1];
+  if (threadIdx.x <= kMaxLevel) side2[threadIdx.x] = geom[6 + threadIdx.x];
+  __syncthreads();
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  // A rope walk visits each node at most once; the bound and the index check
+  // turn a corrupt tree into an error flag instead of a hang or a fault.
+  const int total = (int)min((int64_t)level_off[kMaxLevel + 1], max_nodes);
+  unsigned long long cnt = 0;
+  if (t < n) {
+    const float4 me = pms[t];
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    int i = 0;
+    for (int it = 0; i >= 0; ++it) {
+      if (i >= total || it > total) {
+        atomicOr(flags, 4);
+        break;
+      }
+      const float4 c = com[i];
+      const int4 nd = link[i];
+      const float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
+      const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+      if (side2[link_level(nd.w)] < theta2 * d2) {
+        if (c.w > 0.f && d2 > 0.f) {
+          const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
+          const float s = (c.w * inv) * (inv * inv);
+          ax = fmaf(dx, s, ax);
+          ay = fmaf(dy, s, ay);
+          az = fmaf(dz, s, az);
+          if (kCount) ++cnt;
+        }
+        i = nd.y;
+      } else if (nd.x < 0) {
+        const int e = min(nd.z + link_count(nd.w), n);
+        for (int b = max(nd.z, 0); b < e; ++b) {
+          const float4 q = pms[b];
+          const float bx = q.x - me.x, by = q.y - me.y, bz = q.z - me.z;
+          const float b2 = fmaf(bz, bz, fmaf(by, by, bx * bx));
+          if (b2 > 0.f) {
+            const float inv = __builtin_amdgcn_rsqf(b2 + eps2);
+            const float s = (q.w * inv) * (inv * inv);
+            ax = fmaf(bx, s, ax);
+            ay = fmaf(by, s, ay);
+            az = fmaf(bz, s, az);
+            if (kCount) ++cnt;
+          }
+        }
+        i = nd.y;
+      } else {
+        i = nd.x;
+      }
+    }
+    const uint32_t o = perm[t];
+    if (o < (uint32_t)n) {
+      force[o] = G * (double)ax;
+      force[n + o] = G * (double)ay;
+      force[2 * n + o] = G * (double)az;
+    }
+  }
+  if (kCount) {
+    __shared__ unsigned long long red[256];
+    red[threadIdx.x] = cnt;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(inter, red[0]);
+  }
+}
+
+__global__ void k_zero_u64(unsigned long long* p) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0ull;
+}
+
+int level_bound(int n, int l) {
+  int64_t w = 1;
+  for (int k = 0; k < l && w < n; ++k) w *= 8;
+  return (int)(w < n ? w : n);
+}
+
+// Enqueue an exclusive int scan: out[0..count) from in, count constant or per level.
+hipError_t iscan(const int* in, int* out, int* part, const int* level_off, int l, int n_const,
+                 int bound, int* level_off_mut, hipStream_t s) {
+  const int blocks = (bound + kScanTile - 1) / kScanTile;
+  if (blocks < 1) return hipSuccess;
+  hipLaunchKernelGGL(k_iscan_reduce, dim3(blocks), dim3(kScanThreads), 0, s, in, level_off, l,
+                     n_const, bound, part);
+  hipLaunchKernelGGL(k_iscan_partials, dim3(1), dim3(kScanThreads), 0, s, part, blocks,
+                     level_off_mut, l);
+  hipLaunchKernelGGL(k_iscan_final, dim3(blocks), dim3(kScanThreads), 0, s, in, level_off, l,
+                     n_const, bound, part, out);
+  return hipGetLastError();
+}
+
+}  // namespace
+
+int nb_bh_alloc(nb_ctx* c) {
+  nb_bh_state& b = c->bh;
+  const int64_t n = c->n;
+  int64_t bound = 0, w = 1;
+  for (int l = 0; l <= kMaxLevel; ++l) {
+    bound += w < n ? w : n;
+    if (w < n) w *= 8;
+  }
+  b.max_nodes = bound;
+  b.bbox_blocks = kBboxBlocks;
+  b.sort_blocks = (int)((n + kSortTile - 1) / kSortTile);
+  const int64_t hist = 256 * (int64_t)b.sort_blocks;
+  const int64_t scan_parts = (std::max<int64_t>(hist, n) + kScanTile - 1) / kScanTile + 2;
+  auto A = [](void** p, size_t bytes) { return hipMalloc(p, bytes < 256 ? 256 : bytes); };
+  if (A((void**)&b.bbox_part, sizeof(float) * 6 * kBboxBlocks) ||
+      A((void**)&b.geom, sizeof(float) * 32) ||
+      A((void**)&b.keys[0], sizeof(uint32_t) * n) || A((void**)&b.keys[1], sizeof(uint32_t) * n) ||
+      A((void**)&b.vals[0], sizeof(uint32_t) * n) || A((void**)&b.vals[1], sizeof(uint32_t) * n) ||
+      A((void**)&b.codes, sizeof(uint32_t) * n) ||
+      A((void**)&b.digit_hist, sizeof(int) * 2 * hist) ||
+      A((void**)&b.pms, sizeof(float4) * n) ||
+      A((void**)&b.prefix, sizeof(double4) * (n + 1)) ||
+      A((void**)&b.scan_part, sizeof(double4) * ((n + kScanTile - 1) / kScanTile + 2)) ||
+      A((void**)&b.node_com, sizeof(float4) * bound) ||
+      A((void**)&b.node_link, sizeof(int4) * bound) ||
+      A((void**)&b.level_off, sizeof(int) * (kMaxLevel + 2)) ||
+      A((void**)&b.child_cnt, sizeof(int) * n) || A((void**)&b.child_off, sizeof(int) * n) ||
+      A((void**)&b.iscan_part, sizeof(int) * scan_parts) ||
+      A((void**)&b.counters, sizeof(int) * 16) ||
+      A((void**)&b.inter, sizeof(unsigned long long) * 2))
+    return nb_fail(NB_E_OOM, "nb_create: device allocation failed (octree)");
+  return NB_OK;
+}
+
+void nb_bh_free(nb_ctx* c) {
+  nb_bh_state& b = c->bh;
+  void* ps[] = {b.bbox_part, b.geom,     b.keys[0],  b.keys[1],   b.vals[0],    b.vals[1],
+                b.codes,     b.digit_hist, b.pms,    b.prefix,    b.scan_part,  b.node_com,
+                b.node_link, b.level_off, b.child_cnt, b.child_off, b.iscan_part, b.counters,
+                b.inter};
+  for (void* p : ps)
+    if (p) (void)hipFree(p);
+  b = nb_bh_state{};
+}
+
+hipError_t nb_bh_build(nb_ctx* c) {
+  nb_bh_state& b = c->bh;
+  const int n = c->n;
+  hipStream_t s = c->stream;
+  const int g256 = (n + 255) / 256;
+  // 1. bbox
+  hipLaunchKernelGGL(k_bbox_partial, dim3(b.bbox_blocks), dim3(256), 0, s, c->pm, n, b.bbox_part);
+  hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, s, b.bbox_part, b.bbox_blocks, b.geom);
+  // 2. encode
+  hipLaunchKernelGGL(k_morton, dim3(g256), dim3(256), 0, s, c->pm, n, b.geom, b.codes, b.keys[0],
+                     b.vals[0]);
+  // 3. sort: 4 passes of 8 bits, ping-pong 0 -> 1 -> 0 -> 1 -> 0
+  const int nb = b.sort_blocks;
+  const int hist = 256 * nb;
+  int* offs = b.digit_hist + hist;
+  for (int pass = 0; pass < 4; ++pass) {
+    const int in = pass & 1, out = in ^ 1;
+    hipLaunchKernelGGL(k_radix_hist, dim3(nb), dim3(kSortThreads), 0, s, b.keys[in], n, 8 * pass,
+                       nb, b.digit_hist);
+    hipError_t e = iscan(b.digit_hist, offs, b.iscan_part, nullptr, 0, hist, hist, nullptr, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_radix_scatter, dim3(nb), dim3(kSortThreads), 0, s, b.keys[in],
+                       b.vals[in], n, 8 * pass, nb, offs, b.keys[out], b.vals[out]);
+  }
+  b.sorted_buf = 0;
+  const uint32_t* codes_sorted = b.keys[0];
+  const uint32_t* perm = b.vals[0];
+  // 4. permute
+  hipLaunchKernelGGL(k_gather, dim3(g256), dim3(256), 0, s, c->pm, perm, n, b.pms);
+  // 5. FP64 prefix of mass moments
+  const int mblocks = (n + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(k_mscan_reduce, dim3(mblocks), dim3(kScanThreads), 0, s, b.pms, n, b.scan_part);
+  hipLaunchKernelGGL(k_mscan_partials, dim3(1), dim3(kScanThreads), 0, s, b.scan_part, mblocks);
+  hipLaunchKernelGGL(k_mscan_final, dim3(mblocks), dim3(kScanThreads), 0, s, b.pms, n, b.scan_part,
+                     b.prefix);
+  // 6. octree, level by level
+  hipLaunchKernelGGL(k_tree_init, dim3(1), dim3(64), 0, s, n, b.node_link, b.level_off);
+  const int leaf = c->cfg.leaf_size;
+  for (int l = 0; l < kMaxLevel; ++l) {
+    const int bound = level_bound(n, l);
+    const int grid = std::max(1, std::min((bound + 255) / 256, 4096));
+    hipLaunchKernelGGL(k_tree_count, dim3(grid), dim3(256), 0, s, codes_sorted, n, b.node_link,
+                       b.level_off, l, leaf, bound, b.child_cnt);
+    hipError_t e = iscan(b.child_cnt, b.child_off, b.iscan_part, b.level_off, l, 0, bound,
+                         b.level_off, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_tree_emit, dim3(grid), dim3(256), 0, s, codes_sorted, n, b.node_link,
+                       b.level_off, l, leaf, bound, b.child_off, b.max_nodes, c->flags);
+  }
+  // 7. centre of mass
+  const int cgrid = (int)std::max<int64_t>(1, std::min<int64_t>((b.max_nodes + 255) / 256, 8192));
+  hipLaunchKernelGGL(k_tree_com, dim3(cgrid), dim3(256), 0, s, b.node_link, n, b.max_nodes,
+                     b.level_off, b.prefix,
+                     b.pms, b.node_com);
+  return hipGetLastError();
+}
+
+hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions) {
+  nb_bh_state& b = c->bh;
+  const int n = c->n;
+  const int g256 = (n + 255) / 256;
+  if (count_interactions) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, b.inter);
+    hipLaunchKernelGGL((k_traverse<true>), dim3(g256), dim3(256), 0, c->stream, b.pms, b.vals[0],
+                       n, b.node_com, b.node_link, b.level_off, b.max_nodes, b.geom, c->theta2,
+                       c->cfg.eps2,
+                       c->G, c->force, b.inter, c->flags);
+  } else {
+    hipLaunchKernelGGL((k_traverse<false>), dim3(g256), dim3(256), 0, c->stream, b.pms, b.vals[0],
+                       n, b.node_com, b.node_link, b.level_off, b.max_nodes, b.geom, c->theta2,
+                       c->cfg.eps2,
+                       c->G, c->force, b.inter, c->flags);
+  }
+  return hipGetLastError();
+}
+
+// ---- debug taps ---------------------------------------------------------------------------------
+extern "C" int nb_debug_morton(nb_ctx* c, uint32_t* codes, uint32_t* perm) {
+  if (!c) return nb_fail(NB_E_ARG, "nb_debug_morton: null context");
+  if (c->cfg.solver != NB_SOLVER_BH) return nb_fail(NB_E_ARG, "nb_debug_morton: not a BH context");
+  if (!c->tree_valid) return nb_fail(NB_E_ARG, "nb_debug_morton: no BH force pass has run");
+  NB_HIP(hipStreamSynchronize(c->stream));
+  if (codes) NB_HIP(hipMemcpy(codes, c->bh.codes, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost));
+  if (perm) NB_HIP(hipMemcpy(perm, c->bh.vals[0], sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost));
+  return NB_OK;
+}
+
+extern "C" int nb_debug_tree(nb_ctx* c, nb_node_view* nodes, int64_t cap, int64_t* node_count,
+                             float* bbox) {
+  if (!c) return nb_fail(NB_E_ARG, "nb_debug_tree: null context");
+  if (c->cfg.solver != NB_SOLVER_BH) return nb_fail(NB_E_ARG, "nb_debug_tree: not a BH context");
+  if (!c->tree_valid) return nb_fail(NB_E_ARG, "nb_debug_tree: no BH force pass has run");
+  NB_HIP(hipStreamSynchronize(c->stream));
+  int off[kMaxLevel + 2];
+  NB_HIP(hipMemcpy(off, c->bh.level_off, sizeof(off), hipMemcpyDeviceToHost));
+  const int64_t total = off[kMaxLevel + 1];
+  if (node_count) *node_count = total;
+  if (bbox) NB_HIP(hipMemcpy(bbox, c->bh.geom, sizeof(float) * 6, hipMemcpyDeviceToHost));
+  const int64_t m = std::min<int64_t>(cap, total);
+  if (nodes && m > 0) {
+    std::vector<int4> link(m);
+    std::vector<float4> com(m);
+    NB_HIP(hipMemcpy(link.data(), c->bh.node_link, sizeof(int4) * m, hipMemcpyDeviceToHost));
+    NB_HIP(hipMemcpy(com.data(), c->bh.node_com, sizeof(float4) * m, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < m; ++k) {
+      nb_node_view& v = nodes[k];
+      v.com[0] = com[k].x;
+      v.com[1] = com[k].y;
+      v.com[2] = com[k].z;
+      v.mass = com[k].w;
+      v.first_child = link[k].x;
+      v.next = link[k].y;
+      v.body_start = link[k].z;
+      v.body_count = link[k].w >> 4;
+      v.level = link[k].w & 15;
+      v.pad_ = 0;
+    }
+  }
+  return NB_OK;
+}

@@ -1,0 +1,438 @@
+// nb_capi.hip — the C ABI of nbody.h: context lifetime, device memory,
+// host<->device copies, the step sequence and its hipGraph, diagnostics.
+//
+// Replaces the reference's GL plumbing: SSBO creation at bindings 0-4
+// (ParticleSystem.cpp:73-83, OpenGLBuffer.cpp:12-60), the per-step dispatch
+// sequences of ParticleSolverGPU.cpp:31-41 and ParticleSolverBHutGPU.cpp:25-61,
+// and buffer read-back (OpenGLBuffer.cpp:135-165).
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "nb_internal.h"
+
+static thread_local std::string g_last_error;
+
+void nb_set_error(const std::string& msg) { g_last_error = msg; }
+int nb_fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+extern "C" const char* nb_last_error(void) { return g_last_error.c_str(); }
+
+namespace {
+
+template <typename T>
+hipError_t dalloc(T** p, size_t count) {
+  *p = nullptr;
+  if (count == 0) count = 1;
+  return hipMalloc(reinterpret_cast<void**>(p), count * sizeof(T));
+}
+
+void free_ctx(nb_ctx* c) {
+  if (!c) return;
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  for (auto& e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  nb_bh_free(c);
+  (void)hipFree(c->pos);
+  (void)hipFree(c->vel);
+  (void)hipFree(c->acc);
+  (void)hipFree(c->force);
+  (void)hipFree(c->mass);
+  (void)hipFree(c->pm);
+  (void)hipFree(c->flags);
+  (void)hipFree(c->red);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int check_flags(nb_ctx* c) {
+  int h = 0;
+  NB_HIP(hipMemcpyAsync(&h, c->flags, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(hipStreamSynchronize(c->stream));
+  if (h & 1) return nb_fail(NB_E_NONFINITE, "non-finite position or velocity in the particle state");
+  if (h & 2) return nb_fail(NB_E_TREE_OVERFLOW, "octree node pool exhausted");
+  if (h & 4) return nb_fail(NB_E_TREE_OVERFLOW, "octree traversal left the node range");
+  return NB_OK;
+}
+
+// Default direct-sum launch shape; tuned on MI355X at N = 2^18 (bench.py --tune).
+void default_direct_shape(nb_ctx* c) {
+  c->bpl = c->cfg.bodies_per_lane > 0 ? c->cfg.bodies_per_lane : 2;
+  c->jsplit = c->cfg.j_split > 0 ? c->cfg.j_split : 2;
+  // Small systems: more waves per body block keeps the 256 CUs busy.
+  if (c->cfg.bodies_per_lane <= 0 && c->n_pad < 65536) c->bpl = 1;
+  if (c->cfg.j_split <= 0 && c->n_pad < 65536) c->jsplit = 4;
+}
+
+}  // namespace
+
+hipError_t nb_enqueue_step(nb_ctx* c) {
+  hipError_t e;
+  const bool t = c->timing;
+  if (t) (void)hipEventRecord(c->ev[0], c->stream);
+  if (c->cfg.solver == NB_SOLVER_BH) {
+    e = nb_bh_build(c);
+    if (e != hipSuccess) return e;
+    if (t) (void)hipEventRecord(c->ev[1], c->stream);
+    e = nb_bh_traverse(c, false);
+  } else {
+    if (t) (void)hipEventRecord(c->ev[1], c->stream);
+    e = nb_launch_direct(c);
+  }
+  if (e != hipSuccess) return e;
+  if (t) (void)hipEventRecord(c->ev[2], c->stream);
+  e = nb_launch_integrate(c);
+  if (t) (void)hipEventRecord(c->ev[3], c->stream);
+  return e;
+}
+
+static int accumulate_timing(nb_ctx* c) {
+  NB_HIP(hipEventSynchronize(c->ev[3]));
+  float build = 0.f, force = 0.f, step = 0.f;
+  NB_HIP(hipEventElapsedTime(&build, c->ev[0], c->ev[1]));
+  NB_HIP(hipEventElapsedTime(&force, c->ev[1], c->ev[2]));
+  NB_HIP(hipEventElapsedTime(&step, c->ev[0], c->ev[3]));
+  c->tim.force_ms += force;
+  c->tim.force_calls += 1;
+  c->tim.step_ms += step;
+  c->tim.step_calls += 1;
+  if (c->cfg.solver == NB_SOLVER_BH) {
+    c->tim.build_ms += build;
+    c->tim.build_calls += 1;
+  }
+  return NB_OK;
+}
+
+extern "C" {
+
+int nb_create(nb_ctx** out, const nb_config* cfg) {
+  if (!out || !cfg) return nb_fail(NB_E_ARG, "nb_create: null argument");
+  *out = nullptr;
+  if (cfg->n < 1 || cfg->n >= (int64_t(1) << 30))
+    return nb_fail(NB_E_ARG, "nb_create: n must be in [1, 2^30)");
+  if (!(cfg->eps2 >= 0.f) || !std::isfinite(cfg->eps2))
+    return nb_fail(NB_E_ARG, "nb_create: eps2 must be finite and >= 0");
+  if (!std::isfinite(cfg->dt)) return nb_fail(NB_E_ARG, "nb_create: dt must be finite");
+  if (cfg->solver != NB_SOLVER_DIRECT && cfg->solver != NB_SOLVER_BH)
+    return nb_fail(NB_E_ARG, "nb_create: unknown solver");
+  if (cfg->solver == NB_SOLVER_BH && cfg->n >= (int64_t(1) << 27))
+    return nb_fail(NB_E_ARG, "nb_create: Barnes-Hut supports n < 2^27");
+  if (cfg->integrator != NB_INTEG_KDK_LAGGED && cfg->integrator != NB_INTEG_EULER)
+    return nb_fail(NB_E_ARG, "nb_create: unknown integrator");
+  if (cfg->leaf_size < 0 || cfg->leaf_size > 64)
+    return nb_fail(NB_E_ARG, "nb_create: leaf_size must be in [0, 64]");
+  if (cfg->bodies_per_lane != 0 && cfg->bodies_per_lane != 1 && cfg->bodies_per_lane != 2 &&
+      cfg->bodies_per_lane != 4)
+    return nb_fail(NB_E_ARG, "nb_create: bodies_per_lane must be 0, 1, 2 or 4");
+  if (cfg->j_split != 0 && cfg->j_split != 1 && cfg->j_split != 2 && cfg->j_split != 4)
+    return nb_fail(NB_E_ARG, "nb_create: j_split must be 0, 1, 2 or 4");
+  if (cfg->theta < 0.f || !std::isfinite(cfg->theta))
+    return nb_fail(NB_E_ARG, "nb_create: theta must be finite and >= 0");
+
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return nb_fail(NB_E_NOGPU, "nb_create: no HIP device visible");
+
+  nb_ctx* c = new (std::nothrow) nb_ctx();
+  if (!c) return nb_fail(NB_E_OOM, "nb_create: host allocation failed");
+  c->cfg = *cfg;
+  c->n = (int)cfg->n;
+  c->n_pad = (int)((cfg->n + kPadBodies - 1) / kPadBodies * kPadBodies);
+  c->G = cfg->G == 0.f ? 1.0 : (double)cfg->G;
+  const float theta = cfg->theta == 0.f ? 0.5f : cfg->theta;
+  c->cfg.theta = theta;
+  c->theta2 = theta * theta;
+  if (c->cfg.leaf_size == 0) c->cfg.leaf_size = 8;
+  // The self pair gives dx = 0; its term is exactly 0 unless eps2 is so small
+  // that (eps2)^{-3/2} overflows FP32. Below that, use the guarded kernel.
+  c->guard = !(cfg->eps2 >= 1e-20f);
+  default_direct_shape(c);
+
+  (void)hipGetDevice(&c->device);
+  int rc = NB_OK;
+  auto fail = [&](int code, const std::string& m) {
+    free_ctx(c);
+    return nb_fail(code, m);
+  };
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return fail(NB_E_HIP, "nb_create: hipStreamCreate failed");
+  const size_t n = c->n, np = c->n_pad;
+  if (dalloc(&c->pos, 3 * n) || dalloc(&c->vel, 3 * n) || dalloc(&c->acc, 3 * n) ||
+      dalloc(&c->force, 3 * n) || dalloc(&c->mass, np) || dalloc(&c->pm, np) ||
+      dalloc(&c->flags, 4))
+    return fail(NB_E_OOM, "nb_create: device allocation failed (particle state)");
+  c->red_blocks = 256;
+  if (dalloc(&c->red, 4 * (size_t)c->red_blocks + (np / 256) + 16))
+    return fail(NB_E_OOM, "nb_create: device allocation failed (reductions)");
+  if (hipMemset(c->flags, 0, 4 * sizeof(int)) != hipSuccess ||
+      hipMemset(c->mass, 0, np * sizeof(float)) != hipSuccess ||
+      hipMemset(c->pm, 0, np * sizeof(float4)) != hipSuccess ||
+      hipMemset(c->force, 0, 3 * n * sizeof(double)) != hipSuccess)
+    return fail(NB_E_HIP, "nb_create: hipMemset failed");
+  for (auto& e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) return fail(NB_E_HIP, "nb_create: hipEventCreate failed");
+  if (c->cfg.solver == NB_SOLVER_BH) {
+    rc = nb_bh_alloc(c);
+    if (rc != NB_OK) {
+      std::string m = g_last_error;
+      return fail(rc, m);
+    }
+  }
+  *out = c;
+  return NB_OK;
+}
+
+void nb_destroy(nb_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  free_ctx(ctx);
+}
+
+static int upload_common(nb_ctx* c, const std::vector<double>& p, const std::vector<double>& v,
+                         const std::vector<double>& a, const float* mass) {
+  const size_t n = c->n;
+  // Drop any captured step: it would be replayed against stale assumptions otherwise.
+  if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
+  if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+  NB_HIP(hipMemcpyAsync(c->pos, p.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  NB_HIP(hipMemcpyAsync(c->vel, v.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  NB_HIP(hipMemcpyAsync(c->acc, a.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  NB_HIP(hipMemcpyAsync(c->mass, mass, n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  NB_HIP(nb_launch_fill_i32(c->flags, 0, 4, c->stream));
+  NB_HIP(nb_launch_refresh_pm(c));
+  NB_HIP(hipStreamSynchronize(c->stream));
+  c->uploaded = true;
+  c->tree_valid = false;
+  return NB_OK;
+}
+
+int nb_upload(nb_ctx* c, const float* pos4, const float* vel4, const float* acc4,
+              const float* mass, int64_t n) {
+  if (!c || !pos4 || !vel4 || !mass) return nb_fail(NB_E_ARG, "nb_upload: null argument");
+  if (n != c->n) return nb_fail(NB_E_ARG, "nb_upload: n differs from the context's n");
+  std::vector<double> p(3 * n), v(3 * n), a(3 * n, 0.0);
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      p[k * n + i] = pos4[4 * i + k];
+      v[k * n + i] = vel4[4 * i + k];
+      if (acc4) a[k * n + i] = acc4[4 * i + k];
+    }
+  return upload_common(c, p, v, a, mass);
+}
+
+int nb_upload_f64(nb_ctx* c, const double* pos3, const double* vel3, const double* acc3,
+                  const float* mass, int64_t n) {
+  if (!c || !pos3 || !vel3 || !mass) return nb_fail(NB_E_ARG, "nb_upload_f64: null argument");
+  if (n != c->n) return nb_fail(NB_E_ARG, "nb_upload_f64: n differs from the context's n");
+  std::vector<double> p(3 * n), v(3 * n), a(3 * n, 0.0);
+  for (int64_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      p[k * n + i] = pos3[3 * i + k];
+      v[k * n + i] = vel3[3 * i + k];
+      if (acc3) a[k * n + i] = acc3[3 * i + k];
+    }
+  return upload_common(c, p, v, a, mass);
+}
+
+int nb_step(nb_ctx* c, int32_t nsteps) {
+  if (!c) return nb_fail(NB_E_ARG, "nb_step: null context");
+  if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_step: no particle system uploaded");
+  if (nsteps < 0) return nb_fail(NB_E_ARG, "nb_step: nsteps < 0");
+  if (nsteps == 0) return NB_OK;
+  if (c->timing) {
+    // Timed steps run eagerly with HIP events around each phase.
+    for (int s = 0; s < nsteps; ++s) {
+      NB_HIP(nb_enqueue_step(c));
+      int rc = accumulate_timing(c);
+      if (rc) return rc;
+    }
+    c->tree_valid = c->cfg.solver == NB_SOLVER_BH;
+    return NB_OK;
+  }
+  const bool use_graph = nsteps > 1 && !(c->cfg.flags & NB_F_NO_GRAPH);
+  if (!use_graph) {
+    for (int s = 0; s < nsteps; ++s) NB_HIP(nb_enqueue_step(c));
+  } else {
+    if (!c->graph_exec) {
+      NB_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t e = nb_enqueue_step(c);
+      hipGraph_t g = nullptr;
+      hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+      if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return nb_fail(NB_E_HIP, std::string("nb_step: graph capture failed: ") +
+                                     hipGetErrorString(e != hipSuccess ? e : e2));
+      }
+      c->graph = g;
+      NB_HIP(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+    }
+    for (int s = 0; s < nsteps; ++s) NB_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+  }
+  c->tree_valid = c->cfg.solver == NB_SOLVER_BH;
+  return NB_OK;
+}
+
+int nb_sync(nb_ctx* c) {
+  if (!c) return nb_fail(NB_E_ARG, "nb_sync: null context");
+  NB_HIP(hipStreamSynchronize(c->stream));
+  return check_flags(c);
+}
+
+static int force_pass(nb_ctx* c) {
+  if (!c->uploaded) return nb_fail(NB_E_ARG, "no particle system uploaded");
+  if (c->cfg.solver == NB_SOLVER_BH) {
+    NB_HIP(nb_bh_build(c));
+    NB_HIP(nb_bh_traverse(c, false));
+    c->tree_valid = true;
+  } else {
+    NB_HIP(nb_launch_direct(c));
+  }
+  NB_HIP(hipStreamSynchronize(c->stream));
+  return check_flags(c);
+}
+
+int nb_forces(nb_ctx* c, float* force4) {
+  if (!c || !force4) return nb_fail(NB_E_ARG, "nb_forces: null argument");
+  int rc = force_pass(c);
+  if (rc) return rc;
+  const size_t n = c->n;
+  std::vector<double> f(3 * n);
+  NB_HIP(hipMemcpy(f.data(), c->force, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i) {
+    force4[4 * i] = (float)f[i];
+    force4[4 * i + 1] = (float)f[n + i];
+    force4[4 * i + 2] = (float)f[2 * n + i];
+    force4[4 * i + 3] = 0.f;
+  }
+  return NB_OK;
+}
+
+int nb_forces_f64(nb_ctx* c, double* force3) {
+  if (!c || !force3) return nb_fail(NB_E_ARG, "nb_forces_f64: null argument");
+  int rc = force_pass(c);
+  if (rc) return rc;
+  const size_t n = c->n;
+  std::vector<double> f(3 * n);
+  NB_HIP(hipMemcpy(f.data(), c->force, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) force3[3 * i + k] = f[k * n + i];
+  return NB_OK;
+}
+
+static int download_soa(nb_ctx* c, std::vector<double>& p, std::vector<double>& v,
+                        std::vector<double>& a) {
+  const size_t n = c->n;
+  p.resize(3 * n);
+  v.resize(3 * n);
+  a.resize(3 * n);
+  NB_HIP(hipStreamSynchronize(c->stream));
+  NB_HIP(hipMemcpy(p.data(), c->pos, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+  NB_HIP(hipMemcpy(v.data(), c->vel, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+  NB_HIP(hipMemcpy(a.data(), c->acc, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+  return check_flags(c);
+}
+
+int nb_download(nb_ctx* c, float* pos4, float* vel4, float* acc4) {
+  if (!c) return nb_fail(NB_E_ARG, "nb_download: null context");
+  std::vector<double> p, v, a;
+  const int rc = download_soa(c, p, v, a);
+  const size_t n = c->n;
+  for (size_t i = 0; i < n; ++i)
+    for (int k = 0; k < 4; ++k) {
+      if (pos4) pos4[4 * i + k] = k < 3 ? (float)p[k * n + i] : 0.f;
+      if (vel4) vel4[4 * i + k] = k < 3 ? (float)v[k * n + i] : 0.f;
+      if (acc4) acc4[4 * i + k] = k < 3 ? (float)a[k * n + i] : 0.f;
+    }
+  return rc;
+}
+
+int nb_download_f64(nb_ctx* c, double* pos3, double* vel3, double* acc3) {
+  if (!c) return nb_fail(NB_E_ARG, "nb_download_f64: null context");
+  std::vector<double> p, v, a;
+  const int rc = download_soa(c, p, v, a);
+  const size_t n = c->n;
+  for (size_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) {
+      if (pos3) pos3[3 * i + k] = p[k * n + i];
+      if (vel3) vel3[3 * i + k] = v[k * n + i];
+      if (acc3) acc3[3 * i + k] = a[k * n + i];
+    }
+  return rc;
+}
+
+int nb_energy(nb_ctx* c, double* ekin, double* epot, double p[3]) {
+  if (!c) return nb_fail(NB_E_ARG, "nb_energy: null context");
+  if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_energy: no particle system uploaded");
+  const int kin_n = 4 * c->red_blocks;
+  const int pot_n = (c->n + 255) / 256;
+  NB_HIP(nb_launch_energy(c, c->red, c->red + kin_n));
+  std::vector<double> h(kin_n + pot_n);
+  NB_HIP(hipMemcpyAsync(h.data(), c->red, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(hipStreamSynchronize(c->stream));
+  double ek = 0, px = 0, py = 0, pz = 0, ep = 0;
+  for (int b = 0; b < c->red_blocks; ++b) {
+    ek += h[4 * b];
+    px += h[4 * b + 1];
+    py += h[4 * b + 2];
+    pz += h[4 * b + 3];
+  }
+  for (int b = 0; b < pot_n; ++b) ep += h[kin_n + b];
+  if (ekin) *ekin = ek;
+  if (epot) *epot = -0.5 * c->G * ep;
+  if (p) { p[0] = px; p[1] = py; p[2] = pz; }
+  return check_flags(c);
+}
+
+int nb_set_timing(nb_ctx* c, int32_t enable) {
+  if (!c) return nb_fail(NB_E_ARG, "nb_set_timing: null context");
+  NB_HIP(hipStreamSynchronize(c->stream));
+  c->timing = enable != 0;
+  c->tim = nb_timing{};
+  return NB_OK;
+}
+
+int nb_get_timing(nb_ctx* c, nb_timing* out) {
+  if (!c || !out) return nb_fail(NB_E_ARG, "nb_get_timing: null argument");
+  *out = c->tim;
+  return NB_OK;
+}
+
+int nb_count_interactions(nb_ctx* c, int64_t* interactions) {
+  if (!c || !interactions) return nb_fail(NB_E_ARG, "nb_count_interactions: null argument");
+  if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_count_interactions: nothing uploaded");
+  if (c->cfg.solver != NB_SOLVER_BH) {
+    *interactions = (int64_t)c->n * (c->n - 1);
+    return NB_OK;
+  }
+  NB_HIP(nb_bh_build(c));
+  NB_HIP(nb_bh_traverse(c, true));
+  unsigned long long h = 0;
+  NB_HIP(hipMemcpyAsync(&h, c->bh.inter, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(hipStreamSynchronize(c->stream));
+  *interactions = (int64_t)h;
+  c->tim.interactions = (int64_t)h;
+  return check_flags(c);
+}
+
+int nb_device_info(char* buf, int32_t cap) {
+  if (!buf || cap < 1) return nb_fail(NB_E_ARG, "nb_device_info: bad buffer");
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+    std::snprintf(buf, cap, "no HIP device");
+    return nb_fail(NB_E_NOGPU, "nb_device_info: no HIP device");
+  }
+  std::snprintf(buf, cap, "%s (%s), %d CUs, %.0f MHz, %.1f GiB", prop.name, prop.gcnArchName,
+                prop.multiProcessorCount, prop.clockRate / 1000.0,
+                prop.totalGlobalMem / 1073741824.0);
+  return NB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,110 @@
+// nb_internal.h — device-side state and kernel launchers shared by the
+// translation units of libnbody (not installed; the public ABI is nbody.h).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+#include "nbody.h"
+
+// ---- error plumbing -------------------------------------------------------
+void nb_set_error(const std::string& msg);
+int nb_fail(int code, const std::string& msg);
+#define NB_HIP(call)                                                               \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess)                                                          \
+      return nb_fail(NB_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Padding granule of the FP32 body array: the direct kernel's i-blocks hold
+// 64 lanes x up to 4 bodies, and its j-range is split over up to 4 waves in
+// multiples of 4 bodies, so every count is a multiple of 256.
+constexpr int kPadBodies = 256;
+
+// Octree geometry: 30-bit Morton codes give at most 10 levels below the root.
+constexpr int kMaxLevel = 10;
+
+// Device arrays of the Barnes-Hut pipeline (sized once at nb_create).
+struct nb_bh_state {
+  int64_t max_nodes = 0;
+  // bbox: per-block partial min/max and the final (min xyz, max xyz, side2[11])
+  float* bbox_part = nullptr;   // [2 * 3 * bbox_blocks]
+  float* geom = nullptr;        // [6 + 11]: bmin[3], bmax[3], side^2 per level
+  int bbox_blocks = 0;
+  // Morton keys and body indices, double-buffered for the radix sort
+  uint32_t* keys[2] = {nullptr, nullptr};
+  uint32_t* vals[2] = {nullptr, nullptr};
+  uint32_t* codes = nullptr;    // unsorted codes (debug tap)
+  int* digit_hist = nullptr;       // radix: [256 * sort_blocks] counts, then offsets
+  int sort_blocks = 0;
+  int sorted_buf = 0;           // which of keys[]/vals[] holds the final order
+  float4* pms = nullptr;        // bodies in sorted order (x, y, z, m)
+  double4* prefix = nullptr;    // exclusive FP64 prefix of (m, m x, m y, m z), n+1 entries
+  double4* scan_part = nullptr; // scan block partials
+  // octree, breadth-first
+  float4* node_com = nullptr;   // (cx, cy, cz, m)
+  int4* node_link = nullptr;    // (first_child, next, start, count << 4 | level)
+  int* level_off = nullptr;     // [kMaxLevel + 2] node index where each level begins
+  int* child_cnt = nullptr;     // per node of the current level: non-empty children
+  int* child_off = nullptr;     // exclusive scan of child_cnt
+  int* iscan_part = nullptr;    // int scan block partials
+  int* counters = nullptr;      // misc device counters
+  unsigned long long* inter = nullptr;  // interaction counter (diagnostic traversal)
+};
+
+struct nb_ctx {
+  nb_config cfg{};
+  int n = 0;
+  int n_pad = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  float theta2 = 0.25f;
+  double G = 1.0;
+
+  // particle state, FP64 structure-of-arrays: [x(0..n), y(0..n), z(0..n)]
+  double* pos = nullptr;
+  double* vel = nullptr;
+  double* acc = nullptr;
+  double* force = nullptr;   // last force pass, [3][n]
+  float* mass = nullptr;     // [n_pad], zero in the padding
+  float4* pm = nullptr;      // FP32 force view (x, y, z, m), [n_pad]
+  int* flags = nullptr;      // device status word (bit 0: non-finite state)
+  double* red = nullptr;     // reduction scratch for diagnostics
+  int red_blocks = 0;
+
+  // direct-sum launch shape
+  int bpl = 2;     // i-bodies per lane
+  int jsplit = 2;  // waves per i-block
+  bool guard = false;
+
+  nb_bh_state bh;
+  bool uploaded = false;
+  bool tree_valid = false;
+
+  // one-step hipGraph
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+
+  // timing
+  bool timing = false;
+  hipEvent_t ev[8] = {};
+  nb_timing tim{};
+};
+
+// ---- launchers (return hipError_t of the launch) ---------------------------
+hipError_t nb_launch_direct(nb_ctx* c);  // pm -> force
+hipError_t nb_launch_integrate(nb_ctx* c);  // force + state -> state, pm
+hipError_t nb_launch_refresh_pm(nb_ctx* c);  // pos, mass -> pm
+hipError_t nb_launch_fill_i32(int* p, int v, int64_t count, hipStream_t s);
+hipError_t nb_launch_energy(nb_ctx* c, double* ekin_p, double* epot_p);
+
+// Barnes-Hut pipeline
+int nb_bh_alloc(nb_ctx* c);
+void nb_bh_free(nb_ctx* c);
+hipError_t nb_bh_build(nb_ctx* c);     // bbox, Morton, sort, permute, prefix, octree
+hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions);  // -> force
+
+// step = forces + integration (no sync)
+hipError_t nb_enqueue_step(nb_ctx* c);

@@ -1,0 +1,346 @@
+"""nbody_amd — Python binding (ctypes) of libnbody, the MI355X N-body engine.
+
+This module is a thin host-side mirror of the C ABI in include/nbody.h and
+include/nbody_ic.h; every call goes straight to lib/libnbody.so. There is no
+fallback: if the library is missing, `load()` raises.
+
+The Simulation class follows the reference's ParticleSolver contract
+(include/ParticleSolver.h:8-15): `step()` is updateParticlePositions,
+`squared_softening` is getSquaredSoftening(), `uses_gpu` is usesGPU().
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "lib" / "libnbody.so"
+
+NB_OK = 0
+NB_E_ARG = -1
+NB_E_HIP = -2
+NB_E_OOM = -3
+NB_E_BUDGET = -4
+NB_E_TREE_OVERFLOW = -5
+NB_E_NONFINITE = -6
+NB_E_NOGPU = -7
+
+SOLVER_DIRECT = 0
+SOLVER_BH = 1
+INTEG_KDK_LAGGED = 0
+INTEG_EULER = 1
+F_NO_GRAPH = 0x1
+
+
+class NBodyError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+class Config(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("eps2", C.c_float),
+        ("dt", C.c_float),
+        ("G", C.c_float),
+        ("theta", C.c_float),
+        ("solver", C.c_int32),
+        ("integrator", C.c_int32),
+        ("leaf_size", C.c_int32),
+        ("bodies_per_lane", C.c_int32),
+        ("j_split", C.c_int32),
+        ("flags", C.c_int32),
+    ]
+
+
+class NodeView(C.Structure):
+    _fields_ = [
+        ("com", C.c_float * 3),
+        ("mass", C.c_float),
+        ("first_child", C.c_int32),
+        ("next", C.c_int32),
+        ("body_start", C.c_int32),
+        ("body_count", C.c_int32),
+        ("level", C.c_int32),
+        ("pad_", C.c_int32),
+    ]
+
+
+NODE_DTYPE = np.dtype(
+    [
+        ("com", np.float32, 3),
+        ("mass", np.float32),
+        ("first_child", np.int32),
+        ("next", np.int32),
+        ("body_start", np.int32),
+        ("body_count", np.int32),
+        ("level", np.int32),
+        ("pad_", np.int32),
+    ]
+)
+assert NODE_DTYPE.itemsize == C.sizeof(NodeView)
+
+
+class Timing(C.Structure):
+    _fields_ = [
+        ("force_ms", C.c_double),
+        ("force_calls", C.c_int64),
+        ("step_ms", C.c_double),
+        ("step_calls", C.c_int64),
+        ("build_ms", C.c_double),
+        ("build_calls", C.c_int64),
+        ("interactions", C.c_int64),
+    ]
+
+
+_lib = None
+
+_P = C.c_void_p
+_SIGS = {
+    "nb_create": (C.c_int, [C.POINTER(_P), C.POINTER(Config)]),
+    "nb_destroy": (None, [_P]),
+    "nb_last_error": (C.c_char_p, []),
+    "nb_upload": (C.c_int, [_P, _P, _P, _P, _P, C.c_int64]),
+    "nb_upload_f64": (C.c_int, [_P, _P, _P, _P, _P, C.c_int64]),
+    "nb_step": (C.c_int, [_P, C.c_int32]),
+    "nb_sync": (C.c_int, [_P]),
+    "nb_forces": (C.c_int, [_P, _P]),
+    "nb_forces_f64": (C.c_int, [_P, _P]),
+    "nb_download": (C.c_int, [_P, _P, _P, _P]),
+    "nb_download_f64": (C.c_int, [_P, _P, _P, _P]),
+    "nb_energy": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), _P]),
+    "nb_debug_morton": (C.c_int, [_P, _P, _P]),
+    "nb_debug_tree": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int64), _P]),
+    "nb_set_timing": (C.c_int, [_P, C.c_int32]),
+    "nb_get_timing": (C.c_int, [_P, C.POINTER(Timing)]),
+    "nb_count_interactions": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "nb_device_info": (C.c_int, [C.c_char_p, C.c_int32]),
+    "nb_ic_ball": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, C.c_double, _P, _P, _P]),
+    "nb_ic_plummer": (
+        C.c_int,
+        [C.c_uint64, C.c_int64, C.c_double, C.c_double, C.c_double, C.c_double, _P, _P, _P],
+    ),
+    "nb_ic_discs": (
+        C.c_int,
+        [C.c_uint64, C.c_int64] + [C.c_double] * 8 + [_P, _P, _P],
+    ),
+    "nb_ic_lagrange": (C.c_int, [C.c_double, _P, _P, _P]),
+    "nb_ic_log_uniform_sizes": (C.c_int, [C.c_uint64, C.c_int32, C.c_int32, C.c_int32, _P]),
+    "nb_ic_uniform": (C.c_int, [C.c_uint64, C.c_int32, C.c_double, C.c_double, _P]),
+}
+
+
+def load(path: os.PathLike | str | None = None) -> C.CDLL:
+    """Load libnbody.so (raises FileNotFoundError if it was not built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = Path(path) if path else LIB_PATH
+    if not p.exists():
+        raise FileNotFoundError(f"{p} is missing: build it with `make -C n-body_amd`")
+    lib = C.CDLL(str(p))
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc: int):
+    if rc != NB_OK:
+        raise NBodyError(rc, load().nb_last_error().decode(errors="replace"))
+
+
+def device_info() -> str:
+    buf = C.create_string_buffer(256)
+    rc = load().nb_device_info(buf, 256)
+    _check(rc)
+    return buf.value.decode()
+
+
+# ---- seeded initial conditions -------------------------------------------------
+def ic_ball(seed: int, n: int, radius: float = 100.0, mass_each: float = 1.0):
+    pos = np.empty((n, 3), np.float64)
+    vel = np.empty((n, 3), np.float64)
+    m = np.empty(n, np.float32)
+    _check(load().nb_ic_ball(seed, n, radius, mass_each, _ptr(pos), _ptr(vel), _ptr(m)))
+    return pos, vel, m
+
+
+def ic_plummer(seed: int, n: int, a: float = 50.0, total_mass: float = 1e4,
+               rmax_over_a: float = 10.0, G: float = 1.0):
+    pos = np.empty((n, 3), np.float64)
+    vel = np.empty((n, 3), np.float64)
+    m = np.empty(n, np.float32)
+    _check(load().nb_ic_plummer(seed, n, a, total_mass, rmax_over_a, G, _ptr(pos), _ptr(vel), _ptr(m)))
+    return pos, vel, m
+
+
+def ic_discs(seed: int, n: int, scale_length: float = 30.0, scale_height: float = 2.0,
+             central_factor: float = 1e4, total_mass: float = 1e4, separation: float = 300.0,
+             closing_speed: float = 0.5 / 0.0035, eps2: float = 40.0, G: float = 1.0):
+    pos = np.empty((n, 3), np.float64)
+    vel = np.empty((n, 3), np.float64)
+    m = np.empty(n, np.float32)
+    _check(load().nb_ic_discs(seed, n, scale_length, scale_height, central_factor, total_mass,
+                              separation, closing_speed, eps2, G, _ptr(pos), _ptr(vel), _ptr(m)))
+    return pos, vel, m
+
+
+def ic_lagrange(world_size: float = 5.0):
+    pos = np.empty((3, 3), np.float64)
+    vel = np.empty((3, 3), np.float64)
+    m = np.empty(3, np.float32)
+    _check(load().nb_ic_lagrange(world_size, _ptr(pos), _ptr(vel), _ptr(m)))
+    return pos, vel, m
+
+
+def log_uniform_sizes(seed: int, count: int, lo: int, hi: int) -> np.ndarray:
+    out = np.empty(count, np.int32)
+    _check(load().nb_ic_log_uniform_sizes(seed, count, lo, hi, _ptr(out)))
+    return out
+
+
+def uniform(seed: int, count: int, lo: float, hi: float) -> np.ndarray:
+    out = np.empty(count, np.float64)
+    _check(load().nb_ic_uniform(seed, count, lo, hi, _ptr(out)))
+    return out
+
+
+def to_vec4(a3: np.ndarray) -> np.ndarray:
+    """(n,3) -> (n,4) float32 with w = 0 (the reference's vec4 layout)."""
+    out = np.zeros((a3.shape[0], 4), np.float32)
+    out[:, :3] = a3
+    return out
+
+
+# ---- the solver --------------------------------------------------------------------
+class Simulation:
+    """One device-resident particle system and its solver (an nb_ctx).
+
+    solver: SOLVER_DIRECT (drop-in for the reference's v4) or SOLVER_BH (v7).
+    """
+
+    def __init__(self, n: int, eps2: float, dt: float, solver: int = SOLVER_DIRECT,
+                 integrator: int = INTEG_KDK_LAGGED, G: float = 1.0, theta: float = 0.5,
+                 leaf_size: int = 0, bodies_per_lane: int = 0, j_split: int = 0, flags: int = 0):
+        self._lib = load()
+        self.n = int(n)
+        cfg = Config(n=self.n, eps2=eps2, dt=dt, G=G, theta=theta, solver=solver,
+                     integrator=integrator, leaf_size=leaf_size, bodies_per_lane=bodies_per_lane,
+                     j_split=j_split, flags=flags)
+        self.cfg = cfg
+        self._ctx = C.c_void_p()
+        _check(self._lib.nb_create(C.byref(self._ctx), C.byref(cfg)))
+
+    # ParticleSolver contract
+    @property
+    def squared_softening(self) -> float:
+        return float(self.cfg.eps2)
+
+    uses_gpu = True
+
+    def close(self):
+        if self._ctx:
+            self._lib.nb_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def upload(self, pos, vel, mass, acc=None):
+        """pos/vel/acc: (n,3) float64 or (n,4) float32 (reference vec4 layout); mass: (n,)."""
+        mass = np.ascontiguousarray(mass, np.float32)
+        if pos.shape[1] == 4:
+            pos = np.ascontiguousarray(pos, np.float32)
+            vel = np.ascontiguousarray(vel, np.float32)
+            acc = None if acc is None else np.ascontiguousarray(acc, np.float32)
+            _check(self._lib.nb_upload(self._ctx, _ptr(pos), _ptr(vel), _ptr(acc), _ptr(mass), self.n))
+        else:
+            pos = np.ascontiguousarray(pos, np.float64)
+            vel = np.ascontiguousarray(vel, np.float64)
+            acc = None if acc is None else np.ascontiguousarray(acc, np.float64)
+            _check(self._lib.nb_upload_f64(self._ctx, _ptr(pos), _ptr(vel), _ptr(acc), _ptr(mass), self.n))
+
+    def step(self, nsteps: int = 1):
+        _check(self._lib.nb_step(self._ctx, nsteps))
+
+    def sync(self):
+        _check(self._lib.nb_sync(self._ctx))
+
+    def forces(self) -> np.ndarray:
+        out = np.empty((self.n, 4), np.float32)
+        _check(self._lib.nb_forces(self._ctx, _ptr(out)))
+        return out
+
+    def forces_f64(self) -> np.ndarray:
+        out = np.empty((self.n, 3), np.float64)
+        _check(self._lib.nb_forces_f64(self._ctx, _ptr(out)))
+        return out
+
+    def download(self):
+        p = np.empty((self.n, 4), np.float32)
+        v = np.empty((self.n, 4), np.float32)
+        a = np.empty((self.n, 4), np.float32)
+        _check(self._lib.nb_download(self._ctx, _ptr(p), _ptr(v), _ptr(a)))
+        return p, v, a
+
+    def download_f64(self):
+        p = np.empty((self.n, 3), np.float64)
+        v = np.empty((self.n, 3), np.float64)
+        a = np.empty((self.n, 3), np.float64)
+        _check(self._lib.nb_download_f64(self._ctx, _ptr(p), _ptr(v), _ptr(a)))
+        return p, v, a
+
+    def energy(self):
+        ek = C.c_double()
+        ep = C.c_double()
+        p = np.empty(3, np.float64)
+        _check(self._lib.nb_energy(self._ctx, C.byref(ek), C.byref(ep), _ptr(p)))
+        return ek.value, ep.value, p
+
+    def debug_morton(self):
+        codes = np.empty(self.n, np.uint32)
+        perm = np.empty(self.n, np.uint32)
+        _check(self._lib.nb_debug_morton(self._ctx, _ptr(codes), _ptr(perm)))
+        return codes, perm
+
+    def debug_tree(self):
+        cnt = C.c_int64()
+        bbox = np.empty(6, np.float32)
+        _check(self._lib.nb_debug_tree(self._ctx, None, 0, C.byref(cnt), _ptr(bbox)))
+        nodes = np.empty(cnt.value, NODE_DTYPE)
+        _check(self._lib.nb_debug_tree(self._ctx, _ptr(nodes), cnt.value, C.byref(cnt), _ptr(bbox)))
+        return nodes, bbox
+
+    def set_timing(self, enable: bool):
+        _check(self._lib.nb_set_timing(self._ctx, 1 if enable else 0))
+
+    def timing(self) -> dict:
+        t = Timing()
+        _check(self._lib.nb_get_timing(self._ctx, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Timing._fields_}
+
+    def count_interactions(self) -> int:
+        v = C.c_int64()
+        _check(self._lib.nb_count_interactions(self._ctx, C.byref(v)))
+        return v.value

@@ -1,0 +1,173 @@
+"""GPU parity of the Barnes-Hut engine (drop-in for the reference's v7,
+ParticleSolverBHutGPU.cpp:25-61) against the CPU build of the same pipeline
+(oracle or_bh_*) and the FP64 direct sum, through the C ABI.
+
+Bit-exact: Morton codes, sort permutation, octree topology (first_child, next,
+body range, level). Tolerance: COM 1e-6 relative; forces 1e-5 (p99) relative to
+the CPU build (both FP32, the GPU uses the hardware rsqrt); theta=0.5 error vs
+FP64 no worse than the CPU build's."""
+import numpy as np
+import pytest
+
+from helpers import rel_err
+
+pytestmark = pytest.mark.gpu
+
+EPS2, DT = 40.0, 0.0035
+
+
+def cpu_build(ora, pos4, leaf):
+    mn, mx = ora.bbox(pos4)
+    codes = ora.morton(pos4, mn, mx)
+    perm, srt = ora.sort_codes(codes)
+    tree = ora.build_tree(srt, leaf)
+    com = ora.tree_com(tree, pos4[perm])
+    return mn, mx, codes, perm, tree, com
+
+
+def gpu_run(nb, pos, vel, m, leaf, theta=0.5):
+    with nb.Simulation(len(m), EPS2, DT, solver=nb.SOLVER_BH, leaf_size=leaf, theta=theta) as sim:
+        sim.upload(pos, vel, m)
+        f = sim.forces()
+        codes, perm = sim.debug_morton()
+        nodes, bbox = sim.debug_tree()
+        inter = sim.count_interactions()
+    return f, codes, perm, nodes, bbox, inter
+
+
+def pm4(pos, m):
+    p = np.zeros((len(m), 4), np.float32)
+    p[:, :3] = pos
+    p[:, 3] = m
+    return p
+
+
+@pytest.mark.parametrize("leaf", [1, 4, 16])
+def test_bit_exact_topology_and_forces(nb, ora, leaf):
+    n = 20000
+    pos, vel, m = nb.ic_plummer(300 + leaf, n, 50.0, 1e4)
+    pos[7] = pos[8]  # coincident bodies share a code and a leaf
+    p4 = pm4(pos, m)
+    f, codes, perm, nodes, bbox, inter = gpu_run(nb, pos, vel, m, leaf)
+    mn, mx, ccodes, cperm, tree, com = cpu_build(ora, p4, leaf)
+    np.testing.assert_array_equal(bbox[:3], mn)
+    np.testing.assert_array_equal(bbox[3:], mx)
+    np.testing.assert_array_equal(codes, ccodes)
+    np.testing.assert_array_equal(perm, cperm)
+    assert len(nodes) == len(tree["start"])
+    np.testing.assert_array_equal(nodes["first_child"], tree["first_child"])
+    np.testing.assert_array_equal(nodes["next"], tree["next"])
+    np.testing.assert_array_equal(nodes["body_start"], tree["start"])
+    np.testing.assert_array_equal(nodes["body_count"], tree["count"])
+    np.testing.assert_array_equal(nodes["level"], tree["level"])
+    np.testing.assert_allclose(nodes["mass"], com[:, 3], rtol=1e-6)
+    np.testing.assert_allclose(nodes["com"], com[:, :3], rtol=1e-6, atol=1e-5)
+    fc, cinter = ora.bh_forces(p4[cperm], cperm, mn, mx, tree, com, EPS2)
+    e = rel_err(f, fc)
+    print(f"leaf {leaf}: GPU vs CPU build p50={np.median(e):.2e} p99={np.percentile(e, 99):.2e}; "
+          f"interactions {inter} vs {cinter}")
+    assert np.percentile(e, 99) < 1e-5
+    assert abs(inter - cinter) <= 1e-4 * cinter
+
+
+def test_accuracy_vs_fp64(nb, ora):
+    n = 50000
+    pos, vel, m = nb.ic_plummer(11, n, 50.0, 1e4)
+    f, *_ = gpu_run(nb, pos, vel, m, 8)
+    tgt = np.random.default_rng(1).choice(n, 2000, replace=False)
+    ref = ora.direct_f64(pos.astype(np.float32).astype(np.float64), m.astype(np.float64), EPS2,
+                         targets=tgt)
+    e = rel_err(f[tgt], ref)
+    print(f"BH theta=0.5 N=50000 vs FP64: p50={np.median(e):.2e} p99={np.percentile(e, 99):.2e}")
+    assert np.median(e) < 5e-3 and np.percentile(e, 99) < 3e-2
+
+
+def test_theta_tiny_is_direct_sum(nb, ora):
+    n = 3000
+    pos, vel, m = nb.ic_ball(4, n, 50.0, 1.0)
+    f, *_, inter = gpu_run(nb, pos, vel, m, 8, theta=1e-7)
+    assert inter == n * (n - 1)
+    ref = ora.direct_f64(pos.astype(np.float32).astype(np.float64), m.astype(np.float64), EPS2)
+    assert rel_err(f, ref).max() < 1e-5
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 4097])
+def test_small_and_ragged(nb, ora, n):
+    pos, vel, m = nb.ic_ball(n, n, 10.0, 1.0)
+    f, codes, perm, nodes, bbox, inter = gpu_run(nb, pos, vel, m, 4)
+    p4 = pm4(pos, m)
+    _, _, ccodes, cperm, tree, _ = cpu_build(ora, p4, 4)
+    np.testing.assert_array_equal(codes, ccodes)
+    np.testing.assert_array_equal(perm, cperm)
+    np.testing.assert_array_equal(nodes["next"], tree["next"])
+    assert np.all(np.isfinite(f))
+    if n == 1:
+        assert np.all(f == 0)
+
+
+def test_all_bodies_coincident(nb):
+    n = 100
+    pos = np.tile(np.array([[1.0, 2.0, 3.0]]), (n, 1))
+    f, codes, perm, nodes, bbox, inter = gpu_run(nb, pos, np.zeros((n, 3)), np.ones(n, np.float32), 4)
+    assert np.all(f == 0) and np.all(codes == 0)
+    np.testing.assert_array_equal(perm, np.arange(n))
+    assert nodes["body_count"][-1] == n and nodes["level"][-1] == 10
+
+
+def test_steps_kdk_vs_cpu_pipeline(nb, ora):
+    n = 8192
+    pos, vel, m = nb.ic_plummer(21, n, 30.0, 1e3)
+    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH, leaf_size=8) as sim:
+        sim.upload(pos, vel, m)
+        sim.step(5)
+        p, v, a = sim.download_f64()
+    op, ov, oa = pos.copy(), vel.copy(), np.zeros_like(pos)
+    for _ in range(5):
+        f, *_ = ora.bh_pipeline(pm4(op.astype(np.float32), m), EPS2, theta=0.5, leaf_size=8)
+        ora.kdk_update_f64(op, ov, oa, f[:, :3].astype(np.float64), DT)
+    assert np.abs(p - op).max() < 1e-6 * np.abs(op).max()
+    assert rel_err(a, oa).max() < 1e-3
+
+
+def test_determinism_and_graph(nb):
+    n = 30000
+    pos, vel, m = nb.ic_plummer(31, n, 50.0, 1e4)
+    outs = []
+    for chunks in ([4], [1, 1, 1, 1]):
+        with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH) as sim:
+            sim.upload(pos, vel, m)
+            for k in chunks:
+                sim.step(k)
+            outs.append(sim.download_f64())
+    for x, y in zip(*outs):
+        assert np.array_equal(x, y)
+
+
+def test_c3_discs_properties(nb, ora):
+    n = 1 << 22
+    pos, vel, m = nb.ic_discs(2024, n)
+    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH, leaf_size=8) as sim:
+        sim.upload(pos, vel, m)
+        f = sim.forces_f64()
+        codes, perm = sim.debug_morton()
+        nodes, bbox = sim.debug_tree()
+        sim.step(2)
+        sim.sync()
+    p4 = pm4(pos.astype(np.float32), m)
+    mn, mx = ora.bbox(p4)
+    np.testing.assert_array_equal(bbox[:3], mn)
+    ccodes = ora.morton(p4, mn, mx)
+    np.testing.assert_array_equal(codes, ccodes)
+    cperm, srt = ora.sort_codes(ccodes)
+    np.testing.assert_array_equal(perm, cperm)
+    tree = ora.build_tree(srt, 8)
+    np.testing.assert_array_equal(nodes["first_child"], tree["first_child"])
+    np.testing.assert_array_equal(nodes["next"], tree["next"])
+    np.testing.assert_array_equal(nodes["body_count"], tree["count"])
+    assert abs(nodes["mass"][0] - m.astype(np.float64).sum()) < 1e-5 * m.sum()
+    tgt = np.random.default_rng(5).choice(n, 512, replace=False)
+    ref = ora.direct_f64(pos.astype(np.float32).astype(np.float64), m.astype(np.float64), EPS2,
+                         targets=tgt)
+    e = rel_err(f[tgt], ref)
+    print(f"C3 discs N=2^22 vs FP64 (512 targets): p50={np.median(e):.2e} p99={np.percentile(e, 99):.2e}")
+    assert np.median(e) < 1e-2

@@ -1,0 +1,231 @@
+"""Pins the oracle's Morton encode, sort, octree, COM and Barnes-Hut traversal
+(morton.glsl:37-98; Octree.cpp:9-350) with exhaustive checks, brute force and
+limits, and the v5 restatement with theta -> 0 and monopole limits."""
+import numpy as np
+import pytest
+
+from helpers import mass4, rel_err, seeded_ball, vec4
+
+
+def naive_spread(v):
+    out = 0
+    for b in range(10):
+        out |= ((v >> b) & 1) << (3 * b)
+    return out
+
+
+def test_expand_bits_exhaustive(ora):
+    for v in range(1024):
+        assert ora.expand_bits(v) == naive_spread(v)
+
+
+def test_morton_corners_and_digit_order(ora):
+    pos = vec4(np.array([[0.0, 0, 0], [10.0, 10, 10], [10.0, 0, 0], [0.0, 10, 0], [0.0, 0, 10],
+                         [5.0, 5, 5], [4.999, 0, 0]]))
+    mn, mx = ora.bbox(pos)
+    np.testing.assert_array_equal(mn, [0, 0, 0])
+    np.testing.assert_array_equal(mx, [10, 10, 10])
+    c = ora.morton(pos, mn, mx)
+    assert c[0] == 0
+    assert c[1] == 0x3FFFFFFF  # the max corner clamps to cell 1023 on every axis
+    assert c[2] == naive_spread(1023) << 2  # x is the most significant bit of each triplet
+    assert c[3] == naive_spread(1023) << 1
+    assert c[4] == naive_spread(1023)
+    assert c[5] == naive_spread(512) * 7  # 5/10 * 1024 = 512 on every axis
+    assert c[6] == naive_spread(511) << 2  # floor(0.4999 * 1024) = 511
+
+
+def test_morton_degenerate_extent(ora):
+    # all x equal: extent clamps to 1e-6, (p - min) / 1e-6 = 0 -> cell 0
+    pos = vec4(np.array([[3.0, 0, 0], [3.0, 1, 0], [3.0, 2, 1]]))
+    mn, mx = ora.bbox(pos)
+    c = ora.morton(pos, mn, mx)
+    assert np.all((c >> 2) & naive_spread(1023) == 0)
+
+
+def test_bbox_all_negative_axis(ora):
+    # the reference's max starts at numeric_limits<float>::min() (Octree.cpp:27); the
+    # restatement starts at -FLT_MAX so an all-negative axis keeps its true maximum
+    pos = vec4(np.array([[-3.0, -1, -2], [-1.0, -5, -4]]))
+    mn, mx = ora.bbox(pos)
+    np.testing.assert_array_equal(mx, [-1, -1, -2])
+    np.testing.assert_array_equal(mn, [-3, -5, -4])
+
+
+def test_sort_matches_lexicographic(ora):
+    rng = np.random.default_rng(2)
+    codes = rng.integers(0, 64, 5000).astype(np.uint32)  # many ties
+    perm, srt = ora.sort_codes(codes)
+    ref = np.lexsort((np.arange(5000), codes))
+    np.testing.assert_array_equal(perm, ref)
+    np.testing.assert_array_equal(srt, codes[ref])
+
+
+def brute_tree(codes, leaf):
+    """Recursive digit split, numbered breadth-first afterwards (independent of
+    the oracle's level loop)."""
+    nodes = []  # (level, start, count, children list)
+
+    def split(level, s, e):
+        node = {"level": level, "start": s, "count": e - s, "kids": []}
+        if e - s > leaf and level < 10:
+            shift = 27 - 3 * level
+            b = s
+            while b < e:
+                d = (int(codes[b]) >> shift) & 7
+                c = b
+                while c < e and ((int(codes[c]) >> shift) & 7) == d:
+                    c += 1
+                node["kids"].append(split(level + 1, b, c))
+                b = c
+        return node
+
+    root = split(0, 0, len(codes))
+    order, queue = [], [root]
+    while queue:
+        nd = queue.pop(0)
+        order.append(nd)
+        queue.extend(nd["kids"])
+    idx = {id(nd): i for i, nd in enumerate(order)}
+    nxt = {id(root): -1}
+    for nd in order:
+        for q, k in enumerate(nd["kids"]):
+            nxt[id(k)] = idx[id(nd["kids"][q + 1])] if q + 1 < len(nd["kids"]) else nxt[id(nd)]
+    return {
+        "first_child": np.array([idx[id(nd["kids"][0])] if nd["kids"] else -1 for nd in order]),
+        "next": np.array([nxt[id(nd)] for nd in order]),
+        "start": np.array([nd["start"] for nd in order]),
+        "count": np.array([nd["count"] for nd in order]),
+        "level": np.array([nd["level"] for nd in order]),
+    }
+
+
+@pytest.mark.parametrize("leaf", [1, 3, 8])
+def test_tree_matches_brute_force(ora, leaf):
+    pos = vec4(seeded_ball(700, seed=leaf))
+    pos[10] = pos[11]  # two coincident bodies share a code
+    mn, mx = ora.bbox(pos)
+    perm, srt = ora.sort_codes(ora.morton(pos, mn, mx))
+    t = ora.build_tree(srt, leaf)
+    b = brute_tree(srt, leaf)
+    for k in b:
+        np.testing.assert_array_equal(t[k], b[k], err_msg=k)
+
+
+def rope_walk(t):
+    seen, i = [], 0
+    while i >= 0:
+        seen.append(i)
+        i = t["first_child"][i] if t["first_child"][i] >= 0 else t["next"][i]
+    return seen
+
+
+def test_tree_invariants_and_com(ora):
+    rng = np.random.default_rng(11)
+    pos = vec4(seeded_ball(3000, seed=12))
+    pos[:, 3] = rng.uniform(0.5, 2.0, 3000)
+    mn, mx = ora.bbox(pos)
+    perm, srt = ora.sort_codes(ora.morton(pos, mn, mx))
+    pos_s = pos[perm]
+    t = ora.build_tree(srt, 4)
+    leaves = t["first_child"] < 0
+    # every body in exactly one leaf
+    cover = np.zeros(3000, int)
+    for s, c in zip(t["start"][leaves], t["count"][leaves]):
+        cover[s:s + c] += 1
+    assert np.all(cover == 1)
+    # the rope walk (descend everywhere) visits every node exactly once, in pre-order
+    seen = rope_walk(t)
+    assert sorted(seen) == list(range(len(t["start"])))
+    com = ora.tree_com(t, pos_s)
+    for k in range(0, len(t["start"]), 7):
+        s, c = t["start"][k], t["count"][k]
+        m = pos_s[s:s + c, 3].astype(np.float64)
+        x = pos_s[s:s + c, :3].astype(np.float64)
+        assert abs(com[k, 3] - m.sum()) <= 1e-6 * m.sum()
+        np.testing.assert_allclose(com[k, :3], (m[:, None] * x).sum(0) / m.sum(), rtol=1e-6, atol=1e-5)
+    assert abs(com[0, 3] - pos[:, 3].astype(np.float64).sum()) < 1e-3
+
+
+def test_bh_theta_zero_is_direct(ora):
+    pos = vec4(seeded_ball(800, seed=13))
+    pos[:, 3] = 1.0
+    f, codes, perm, nodes, inter = ora.bh_pipeline(pos, 4.0, theta=1e-7, leaf_size=4)
+    ref = ora.direct_f64(pos[:, :3].astype(np.float64), np.ones(800), 4.0)
+    assert inter == 800 * 799
+    assert rel_err(f, ref).max() < 1e-5
+
+
+def test_bh_distant_cluster_monopole(ora):
+    # Cluster B (16 bodies within radius 1) seen from cluster A 1000 units away is
+    # accepted as one cell at theta = 0.5: force = M_B (c_B - x) / (d^2 + eps2)^1.5.
+    rng = np.random.default_rng(14)
+    a = rng.normal(scale=0.5, size=(16, 3))
+    b = rng.normal(scale=0.5, size=(16, 3)) + np.array([1000.0, 0, 0])
+    pos = vec4(np.vstack([a, b]))
+    pos[:, 3] = 1.0
+    f, _, _, _, _ = ora.bh_pipeline(pos, 0.01, theta=0.5, leaf_size=1)
+    cb = b.astype(np.float32).astype(np.float64).mean(0)
+    for i in range(16):
+        d = cb - pos[i, :3].astype(np.float64)
+        r2 = d @ d + 0.01
+        mono = 16 * d / r2 ** 1.5
+        near = ora.direct_f64(a.astype(np.float32).astype(np.float64), np.ones(16), 0.01)[i]
+        np.testing.assert_allclose(f[i, :3], near + mono, rtol=1e-4, atol=1e-9)
+
+
+def test_bh_cpu_build_accuracy(ora):
+    # theta = 0.5 monopole error of the engine's tree is no worse than the v5
+    # restatement's on the same input (its leaves are evaluated body by body).
+    pos = vec4(seeded_ball(4000, seed=15))
+    pos[:, 3] = 1.0
+    ref = ora.direct_f64(pos[:, :3].astype(np.float64), np.ones(4000), 40.0)
+    rc, f5, _ = ora.v5_forces(pos, mass4(np.ones(4000)), 40.0, theta=0.5)
+    e5 = rel_err(f5, ref)
+    for leaf in (1, 8):
+        f, _, _, _, inter = ora.bh_pipeline(pos, 40.0, theta=0.5, leaf_size=leaf)
+        e = rel_err(f, ref)
+        assert np.median(e) < 1e-2 and np.percentile(e, 99) < 3e-2
+        assert np.median(e) <= 1.02 * np.median(e5)
+        assert inter < 4000 * 3999 / 5
+
+
+def test_v5_theta_zero_is_direct(ora):
+    pos = vec4(seeded_ball(600, seed=16))
+    m = np.random.default_rng(17).uniform(0.5, 2.0, 600)
+    rc, f, used = ora.v5_forces(pos, mass4(m), 4.0, theta=1e-7)
+    assert rc == 0 and used > 600
+    ref = ora.direct_f64(pos[:, :3].astype(np.float64), m.astype(np.float32).astype(np.float64), 4.0)
+    assert rel_err(f, ref).max() < 1e-5
+
+
+def test_v5_accuracy_and_agreement(ora):
+    pos = vec4(seeded_ball(3000, seed=18))
+    mm = mass4(np.ones(3000))
+    rc, f5, _ = ora.v5_forces(pos, mm, 40.0, theta=0.5)
+    assert rc == 0
+    ref = ora.direct_f64(pos[:, :3].astype(np.float64), np.ones(3000), 40.0)
+    e5 = rel_err(f5, ref)
+    assert np.median(e5) < 1e-2
+    pm = pos.copy()
+    pm[:, 3] = 1.0  # the engine's layout carries the mass in w
+    fb, *_ = ora.bh_pipeline(pm, 40.0, theta=0.5, leaf_size=1)
+    # same opening rule and cell geometry (up to quantisation): forces agree closely
+    assert np.median(rel_err(fb, f5)) < 5e-3
+
+
+def test_v5_monopole_and_overflow(ora):
+    rng = np.random.default_rng(19)
+    a = rng.normal(scale=0.5, size=(8, 3))
+    b = rng.normal(scale=0.5, size=(8, 3)) + np.array([0, 500.0, 0])
+    pos = vec4(np.vstack([a, b]))
+    rc, f, _ = ora.v5_forces(pos, mass4(np.ones(16)), 0.01, theta=0.5)
+    assert rc == 0
+    cb = b.astype(np.float32).astype(np.float64).mean(0)
+    d = cb - pos[0, :3].astype(np.float64)
+    mono = 8 * d / (d @ d + 0.01) ** 1.5
+    near = ora.direct_f64(a.astype(np.float32).astype(np.float64), np.ones(8), 0.01)[0]
+    np.testing.assert_allclose(f[0, :3], near + mono, rtol=1e-4, atol=1e-9)
+    # a pool of one node per body overflows: reported, not silent (reference D5)
+    rc, _, _ = ora.v5_forces(pos, mass4(np.ones(16)), 0.01, theta=0.5, pool_factor=1.0)
+    assert rc == -5
