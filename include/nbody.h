@@ -110,6 +110,14 @@ int nb_download_f64(nb_ctx* ctx, double* pos3, double* vel3, double* acc3);
  * epot = -G sum_{i<j} m_i m_j / sqrt(r^2 + eps2). */
 int nb_energy(nb_ctx* ctx, double* ekin, double* epot, double p[3]);
 
+/* Binary snapshot of the device state: n, eps2, dt, FP64 positions, velocities,
+ * accelerations and FP32 masses. Loading it into a context with the same n and
+ * continuing reproduces an uninterrupted run bit for bit. The reference's only
+ * persistence is the text format (ParticleSystem.cpp:205-215 writer,
+ * ParticleSystemFile.cpp:10-90 reader). */
+int nb_save_snapshot(nb_ctx* ctx, const char* path);
+int nb_load_snapshot(nb_ctx* ctx, const char* path);
+
 /* ---- Barnes-Hut debug taps (valid after a BH nb_step or nb_forces) ---- */
 /* codes[n]: 30-bit Morton code of body i (unsorted order);
  * perm[n]: body index at sorted position k (sorted by (code, index)). */

@@ -2,7 +2,8 @@
  * nbody_ic.h — seeded initial conditions (host-only, part of libnbody).
  *
  * Counterpart of the reference's ParticleSystemInitializer::generateParticles
- * (include/ParticleSystemInitializer.h; src/ParticleSystemInitializer/*), whose
+ * (include/ParticleSystemInitializer.h and the classes under
+ * src/ParticleSystemInitializer/), whose
  * random initializers are unseeded (ParticleSystemBall.cpp:15-16,
  * ParticleSystemGalaxyInitializer.cpp:17-18). Here every generator is a pure
  * function of (seed, parameters): a SplitMix64 stream gives uniform doubles in
@@ -45,6 +46,20 @@ int nb_ic_discs(uint64_t seed, int64_t n, double scale_length, double scale_heig
 /* The reference's deterministic Lagrange triangle, 3 bodies
  * (ParticleSystemLagrange.cpp:8-50), computed in single precision as there. */
 int nb_ic_lagrange(double world_size, double* pos3, double* vel3, float* mass);
+
+/* Reference particle-file format (ParticleSystemFile.cpp:10-90):
+ *   "Particle System with N particles:" then per body "Particle ID: i",
+ *   "Position: (x, y, z)", "Velocity: (...)", "Acceleration: (...)", "Mass: m".
+ * A body is emitted at each "Mass:" line with the most recent position,
+ * velocity and acceleration; reading stops after N bodies. Returns the number of
+ * bodies read (writing at most cap of them; pass cap = 0 to count), or NB_E_ARG
+ * when the file cannot be opened or has no header (the reference exits). */
+int64_t nb_ic_read_file(const char* path, int64_t cap, float* pos4, float* vel4, float* acc4,
+                        float* mass);
+/* Writes the same format with 9 significant digits (every float round-trips;
+ * the reference's writer uses the stream default of 6, ParticleSystem.cpp:205-215). */
+int nb_ic_write_file(const char* path, int64_t n, const float* pos4, const float* vel4,
+                     const float* acc4, const float* mass);
 
 /* Log-uniform integers floor(exp(U(ln lo, ln hi))) in [lo, hi]. */
 int nb_ic_log_uniform_sizes(uint64_t seed, int32_t count, int32_t lo, int32_t hi, int32_t* out);

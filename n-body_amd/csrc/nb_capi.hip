@@ -390,6 +390,61 @@ int nb_energy(nb_ctx* c, double* ekin, double* epot, double p[3]) {
   return check_flags(c);
 }
 
+// Snapshot layout (little-endian): "NBSNAP01", int64 n, float eps2, float dt,
+// int32 integrator, int32 solver, then the device's structure-of-arrays buffers
+// as stored: positions, velocities, accelerations (double[3][n] each) and
+// masses (float[n]).
+static const char kSnapMagic[8] = {'N', 'B', 'S', 'N', 'A', 'P', '0', '1'};
+
+int nb_save_snapshot(nb_ctx* c, const char* path) {
+  if (!c || !path) return nb_fail(NB_E_ARG, "nb_save_snapshot: null argument");
+  if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_save_snapshot: nothing uploaded");
+  std::vector<double> p, v, a;
+  int rc = download_soa(c, p, v, a);
+  if (rc) return rc;
+  std::vector<float> m(c->n);
+  NB_HIP(hipMemcpy(m.data(), c->mass, sizeof(float) * c->n, hipMemcpyDeviceToHost));
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return nb_fail(NB_E_ARG, std::string("nb_save_snapshot: cannot open ") + path);
+  const int64_t n = c->n;
+  const int32_t meta[2] = {c->cfg.integrator, c->cfg.solver};
+  bool ok = std::fwrite(kSnapMagic, 1, 8, f) == 8 && std::fwrite(&n, 8, 1, f) == 1 &&
+            std::fwrite(&c->cfg.eps2, 4, 1, f) == 1 && std::fwrite(&c->cfg.dt, 4, 1, f) == 1 &&
+            std::fwrite(meta, 4, 2, f) == 2 &&
+            std::fwrite(p.data(), 8, p.size(), f) == p.size() &&
+            std::fwrite(v.data(), 8, v.size(), f) == v.size() &&
+            std::fwrite(a.data(), 8, a.size(), f) == a.size() &&
+            std::fwrite(m.data(), 4, m.size(), f) == m.size();
+  ok = (std::fclose(f) == 0) && ok;
+  return ok ? NB_OK : nb_fail(NB_E_ARG, std::string("nb_save_snapshot: write failed: ") + path);
+}
+
+int nb_load_snapshot(nb_ctx* c, const char* path) {
+  if (!c || !path) return nb_fail(NB_E_ARG, "nb_load_snapshot: null argument");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return nb_fail(NB_E_ARG, std::string("nb_load_snapshot: cannot open ") + path);
+  char magic[8];
+  int64_t n = 0;
+  float eps2 = 0, dt = 0;
+  int32_t meta[2];
+  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kSnapMagic, 8) == 0 &&
+            std::fread(&n, 8, 1, f) == 1 && std::fread(&eps2, 4, 1, f) == 1 &&
+            std::fread(&dt, 4, 1, f) == 1 && std::fread(meta, 4, 2, f) == 2;
+  if (!ok || n != c->n) {
+    std::fclose(f);
+    return nb_fail(NB_E_ARG, "nb_load_snapshot: not a snapshot of this context's size");
+  }
+  std::vector<double> p(3 * n), v(3 * n), a(3 * n);
+  std::vector<float> m(n);
+  ok = std::fread(p.data(), 8, p.size(), f) == p.size() &&
+       std::fread(v.data(), 8, v.size(), f) == v.size() &&
+       std::fread(a.data(), 8, a.size(), f) == a.size() &&
+       std::fread(m.data(), 4, m.size(), f) == m.size();
+  std::fclose(f);
+  if (!ok) return nb_fail(NB_E_ARG, "nb_load_snapshot: truncated file");
+  return upload_common(c, p, v, a, m.data());
+}
+
 int nb_set_timing(nb_ctx* c, int32_t enable) {
   if (!c) return nb_fail(NB_E_ARG, "nb_set_timing: null context");
   NB_HIP(hipStreamSynchronize(c->stream));
