@@ -155,6 +155,47 @@ int nb_get_timing(nb_ctx* ctx, nb_timing* out);
  * traversal, not on the step path). */
 int nb_count_interactions(nb_ctx* ctx, int64_t* interactions);
 
+/* ---- batched ensembles ---------------------------------------------------
+ * Many independent particle systems ("clusters") of uneven size, each with its
+ * own eps2, dt and G, advanced together: one persistent force launch takes
+ * (cluster, body-block) work items from a device work queue, one launch
+ * integrates every cluster, and multi-step calls replay a captured hipGraph.
+ * Clusters live in one HBM body pool (max_bodies, each cluster padded to a
+ * multiple of 256) and can be added and removed between steps; a cluster's
+ * results do not depend on which other clusters share the launch (fixed
+ * per-cluster summation order, no cross-cluster tiles).
+ * The reference has no counterpart (it simulates one system per process,
+ * ParticleSimulation.cpp:4-14); this is the engine's batching extension. */
+typedef struct nb_batch nb_batch;
+typedef struct nb_batch_config {
+  int64_t max_bodies;      /* body pool capacity */
+  int32_t max_clusters;
+  int32_t integrator;      /* nb_integrator, shared by all clusters */
+  int32_t bodies_per_lane; /* 0 = tuned default */
+  int32_t j_split;         /* 0 = tuned default */
+  int32_t flags;           /* NB_F_NO_GRAPH */
+  int32_t pad_;
+} nb_batch_config;
+int nb_batch_create(nb_batch** out, const nb_batch_config* cfg);
+void nb_batch_destroy(nb_batch* b);
+/* Admit a cluster (FP64 state, interleaved xyz; acc3 may be NULL). *id receives
+ * its slot. NB_E_BUDGET when the pool or the slot table cannot hold it now. */
+int nb_batch_add(nb_batch* b, const double* pos3, const double* vel3, const double* acc3,
+                 const float* mass, int64_t n, float eps2, float dt, float G, int32_t* id);
+int nb_batch_remove(nb_batch* b, int32_t id);
+int nb_batch_step(nb_batch* b, int32_t nsteps);
+int nb_batch_sync(nb_batch* b);
+/* Force-only pass for every cluster (state untouched); writes cluster id's
+ * forces to force3 (double[3n], interleaved). */
+int nb_batch_forces(nb_batch* b, int32_t id, double* force3);
+int nb_batch_download(nb_batch* b, int32_t id, double* pos3, double* vel3, double* acc3);
+/* FP64 kinetic and softened potential energy of one cluster. */
+int nb_batch_energy(nb_batch* b, int32_t id, double* ekin, double* epot);
+/* Active clusters, pool bodies in use (padded), pair interactions per step. */
+int nb_batch_info(nb_batch* b, int32_t* clusters, int64_t* bodies, int64_t* interactions);
+int nb_batch_set_timing(nb_batch* b, int32_t enable);
+int nb_batch_get_timing(nb_batch* b, nb_timing* out);
+
 /* Library / device description for logs: writes a NUL-terminated string. */
 int nb_device_info(char* buf, int32_t cap);
 

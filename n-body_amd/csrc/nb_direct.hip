@@ -14,23 +14,14 @@
 //   * One interaction = 3 v_sub + 3 v_fma (r^2) + v_rsq + 3 v_mul + 3 v_fma.
 //     The self pair needs no branch: dx = 0 contributes exactly 0 when eps2 > 0.
 //   * Compensated accumulation: plain FP32 partial sums over a chunk of
-//     kChunk sources, folded into a (sum, error) pair with Knuth's TwoSum once
+//     256 sources, folded into a (sum, error) pair with Knuth's TwoSum once
 //     per chunk, so compensation costs < 1% of the ALU work. The K wave partials
 //     are merged in LDS in fixed wave order and widened to FP64; G is applied in
 //     FP64. Results are deterministic (no atomics, fixed order).
+#include "nb_force_core.h"
 #include "nb_internal.h"
 
 namespace {
-
-constexpr int kChunk = 256;  // sources per compensated fold
-
-__device__ __forceinline__ void two_sum_fold(float& s, float& c, float x) {
-  const float t = s + x;
-  const float bp = t - s;
-  const float e = (s - (t - bp)) + (x - bp);
-  s = t;
-  c += e;
-}
 
 template <int B, int K, bool kGuard>
 __global__ __launch_bounds__(64 * K) void k_direct(const float4* __restrict__ pm, int n,
@@ -39,95 +30,22 @@ __global__ __launch_bounds__(64 * K) void k_direct(const float4* __restrict__ pm
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int ibase = blockIdx.x * (64 * B) + lane;
-
-  float xi[B], yi[B], zi[B];
-  float sx[B], sy[B], sz[B], cx[B], cy[B], cz[B];
-#pragma unroll
-  for (int b = 0; b < B; ++b) {
-    const float4 p = pm[ibase + 64 * b];
-    xi[b] = p.x;
-    yi[b] = p.y;
-    zi[b] = p.z;
-    sx[b] = sy[b] = sz[b] = 0.f;
-    cx[b] = cy[b] = cz[b] = 0.f;
-  }
-
+  nbcore::Targets<B> t;
+  nbcore::load_targets<B>(pm, ibase, t);
   const int jlen = n_pad / K;  // multiple of 64 (n_pad is a multiple of 256)
-  const int j0 = wave * jlen;
-  const int j1 = j0 + jlen;
-  for (int jc = j0; jc < j1; jc += kChunk) {
-    const int je = min(jc + kChunk, j1);
-    float px[B], py[B], pz[B];
-#pragma unroll
-    for (int b = 0; b < B; ++b) px[b] = py[b] = pz[b] = 0.f;
-    for (int j = jc; j < je; j += 4) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const float4 q = pm[j + u];  // wave-uniform: scalar load
-#pragma unroll
-        for (int b = 0; b < B; ++b) {
-          const float dx = q.x - xi[b];
-          const float dy = q.y - yi[b];
-          const float dz = q.z - zi[b];
-          const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-          const float inv = __builtin_amdgcn_rsqf(r2);
-          float s = (q.w * inv) * (inv * inv);
-          if (kGuard) s = r2 > 0.f ? s : 0.f;
-          px[b] = fmaf(dx, s, px[b]);
-          py[b] = fmaf(dy, s, py[b]);
-          pz[b] = fmaf(dz, s, pz[b]);
-        }
-      }
-    }
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-      two_sum_fold(sx[b], cx[b], px[b]);
-      two_sum_fold(sy[b], cy[b], py[b]);
-      two_sum_fold(sz[b], cz[b], pz[b]);
-    }
-  }
-
+  nbcore::accumulate<B, kGuard>(pm, wave * jlen, wave * jlen + jlen, eps2, t);
   if constexpr (K > 1) {
-    // Merge the K wave partials in wave order 0..K-1.
     __shared__ float part[K][6][B][64];
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-      part[wave][0][b][lane] = sx[b];
-      part[wave][1][b][lane] = sy[b];
-      part[wave][2][b][lane] = sz[b];
-      part[wave][3][b][lane] = cx[b];
-      part[wave][4][b][lane] = cy[b];
-      part[wave][5][b][lane] = cz[b];
-    }
-    __syncthreads();
+    nbcore::merge_waves<B, K>(part, wave, lane, t);
     if (wave != 0) return;
-#pragma unroll
-    for (int b = 0; b < B; ++b) {
-      sx[b] = part[0][0][b][lane];
-      sy[b] = part[0][1][b][lane];
-      sz[b] = part[0][2][b][lane];
-      cx[b] = part[0][3][b][lane];
-      cy[b] = part[0][4][b][lane];
-      cz[b] = part[0][5][b][lane];
-#pragma unroll
-      for (int w = 1; w < K; ++w) {
-        two_sum_fold(sx[b], cx[b], part[w][0][b][lane]);
-        two_sum_fold(sy[b], cy[b], part[w][1][b][lane]);
-        two_sum_fold(sz[b], cz[b], part[w][2][b][lane]);
-        cx[b] += part[w][3][b][lane];
-        cy[b] += part[w][4][b][lane];
-        cz[b] += part[w][5][b][lane];
-      }
-    }
   }
-
 #pragma unroll
   for (int b = 0; b < B; ++b) {
     const int i = ibase + 64 * b;
     if (i < n) {
-      force[i] = G * ((double)sx[b] + (double)cx[b]);
-      force[n + i] = G * ((double)sy[b] + (double)cy[b]);
-      force[2 * n + i] = G * ((double)sz[b] + (double)cz[b]);
+      force[i] = G * ((double)t.sx[b] + (double)t.cx[b]);
+      force[n + i] = G * ((double)t.sy[b] + (double)t.cy[b]);
+      force[2 * n + i] = G * ((double)t.sz[b] + (double)t.cz[b]);
     }
   }
 }

@@ -97,6 +97,18 @@ class Timing(C.Structure):
     ]
 
 
+class BatchConfig(C.Structure):
+    _fields_ = [
+        ("max_bodies", C.c_int64),
+        ("max_clusters", C.c_int32),
+        ("integrator", C.c_int32),
+        ("bodies_per_lane", C.c_int32),
+        ("j_split", C.c_int32),
+        ("flags", C.c_int32),
+        ("pad_", C.c_int32),
+    ]
+
+
 _lib = None
 
 _P = C.c_void_p
@@ -113,12 +125,27 @@ _SIGS = {
     "nb_download": (C.c_int, [_P, _P, _P, _P]),
     "nb_download_f64": (C.c_int, [_P, _P, _P, _P]),
     "nb_energy": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double), _P]),
+    "nb_save_snapshot": (C.c_int, [_P, C.c_char_p]),
+    "nb_load_snapshot": (C.c_int, [_P, C.c_char_p]),
     "nb_debug_morton": (C.c_int, [_P, _P, _P]),
     "nb_debug_tree": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int64), _P]),
     "nb_set_timing": (C.c_int, [_P, C.c_int32]),
     "nb_get_timing": (C.c_int, [_P, C.POINTER(Timing)]),
     "nb_count_interactions": (C.c_int, [_P, C.POINTER(C.c_int64)]),
     "nb_device_info": (C.c_int, [C.c_char_p, C.c_int32]),
+    "nb_batch_create": (C.c_int, [C.POINTER(_P), C.POINTER(BatchConfig)]),
+    "nb_batch_destroy": (None, [_P]),
+    "nb_batch_add": (C.c_int, [_P, _P, _P, _P, _P, C.c_int64, C.c_float, C.c_float, C.c_float,
+                               C.POINTER(C.c_int32)]),
+    "nb_batch_remove": (C.c_int, [_P, C.c_int32]),
+    "nb_batch_step": (C.c_int, [_P, C.c_int32]),
+    "nb_batch_sync": (C.c_int, [_P]),
+    "nb_batch_forces": (C.c_int, [_P, C.c_int32, _P]),
+    "nb_batch_download": (C.c_int, [_P, C.c_int32, _P, _P, _P]),
+    "nb_batch_energy": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+    "nb_batch_info": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "nb_batch_set_timing": (C.c_int, [_P, C.c_int32]),
+    "nb_batch_get_timing": (C.c_int, [_P, C.POINTER(Timing)]),
     "nb_ic_ball": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, C.c_double, _P, _P, _P]),
     "nb_ic_plummer": (
         C.c_int,
@@ -129,6 +156,8 @@ _SIGS = {
         [C.c_uint64, C.c_int64] + [C.c_double] * 8 + [_P, _P, _P],
     ),
     "nb_ic_lagrange": (C.c_int, [C.c_double, _P, _P, _P]),
+    "nb_ic_read_file": (C.c_int64, [C.c_char_p, C.c_int64, _P, _P, _P, _P]),
+    "nb_ic_write_file": (C.c_int, [C.c_char_p, C.c_int64, _P, _P, _P, _P]),
     "nb_ic_log_uniform_sizes": (C.c_int, [C.c_uint64, C.c_int32, C.c_int32, C.c_int32, _P]),
     "nb_ic_uniform": (C.c_int, [C.c_uint64, C.c_int32, C.c_double, C.c_double, _P]),
 }
@@ -203,6 +232,23 @@ def ic_lagrange(world_size: float = 5.0):
     m = np.empty(3, np.float32)
     _check(load().nb_ic_lagrange(world_size, _ptr(pos), _ptr(vel), _ptr(m)))
     return pos, vel, m
+
+
+def read_particle_file(path):
+    """Reference text format (ParticleSystemFile.cpp:10-90) -> pos4, vel4, acc4, mass."""
+    lib = load()
+    n = lib.nb_ic_read_file(str(path).encode(), 0, None, None, None, None)
+    if n < 0:
+        raise NBodyError(int(n), f"cannot read particle file {path}")
+    p, v, a = (np.zeros((n, 4), np.float32) for _ in range(3))
+    m = np.zeros(n, np.float32)
+    lib.nb_ic_read_file(str(path).encode(), n, _ptr(p), _ptr(v), _ptr(a), _ptr(m))
+    return p, v, a, m
+
+
+def write_particle_file(path, pos4, vel4, acc4, mass):
+    arrs = [np.ascontiguousarray(x, np.float32) for x in (pos4, vel4, acc4, mass)]
+    _check(load().nb_ic_write_file(str(path).encode(), arrs[3].shape[0], *(_ptr(x) for x in arrs)))
 
 
 def log_uniform_sizes(seed: int, count: int, lo: int, hi: int) -> np.ndarray:
@@ -318,6 +364,12 @@ class Simulation:
         _check(self._lib.nb_energy(self._ctx, C.byref(ek), C.byref(ep), _ptr(p)))
         return ek.value, ep.value, p
 
+    def save_snapshot(self, path):
+        _check(self._lib.nb_save_snapshot(self._ctx, str(path).encode()))
+
+    def load_snapshot(self, path):
+        _check(self._lib.nb_load_snapshot(self._ctx, str(path).encode()))
+
     def debug_morton(self):
         codes = np.empty(self.n, np.uint32)
         perm = np.empty(self.n, np.uint32)
@@ -344,3 +396,95 @@ class Simulation:
         v = C.c_int64()
         _check(self._lib.nb_count_interactions(self._ctx, C.byref(v)))
         return v.value
+
+
+class Batch:
+    """Many independent clusters in one persistent launch (nb_batch_*)."""
+
+    def __init__(self, max_bodies: int, max_clusters: int, integrator: int = INTEG_KDK_LAGGED,
+                 bodies_per_lane: int = 0, j_split: int = 0, flags: int = 0):
+        self._lib = load()
+        cfg = BatchConfig(max_bodies=max_bodies, max_clusters=max_clusters, integrator=integrator,
+                          bodies_per_lane=bodies_per_lane, j_split=j_split, flags=flags)
+        self._b = C.c_void_p()
+        _check(self._lib.nb_batch_create(C.byref(self._b), C.byref(cfg)))
+        self.sizes: dict[int, int] = {}
+
+    def close(self):
+        if self._b:
+            self._lib.nb_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add(self, pos, vel, mass, eps2: float, dt: float, G: float = 1.0, acc=None) -> int:
+        pos = np.ascontiguousarray(pos, np.float64)
+        vel = np.ascontiguousarray(vel, np.float64)
+        mass = np.ascontiguousarray(mass, np.float32)
+        acc = None if acc is None else np.ascontiguousarray(acc, np.float64)
+        cid = C.c_int32()
+        _check(self._lib.nb_batch_add(self._b, _ptr(pos), _ptr(vel), _ptr(acc), _ptr(mass),
+                                      mass.shape[0], eps2, dt, G, C.byref(cid)))
+        self.sizes[cid.value] = mass.shape[0]
+        return cid.value
+
+    def try_add(self, *a, **k):
+        """add() that returns None instead of raising when the pool is full."""
+        try:
+            return self.add(*a, **k)
+        except NBodyError as e:
+            if e.code == NB_E_BUDGET:
+                return None
+            raise
+
+    def remove(self, cid: int):
+        _check(self._lib.nb_batch_remove(self._b, cid))
+        self.sizes.pop(cid, None)
+
+    def step(self, nsteps: int = 1):
+        _check(self._lib.nb_batch_step(self._b, nsteps))
+
+    def sync(self):
+        _check(self._lib.nb_batch_sync(self._b))
+
+    def forces(self, cid: int) -> np.ndarray:
+        out = np.empty((self.sizes[cid], 3), np.float64)
+        _check(self._lib.nb_batch_forces(self._b, cid, _ptr(out)))
+        return out
+
+    def download(self, cid: int):
+        n = self.sizes[cid]
+        p, v, a = (np.empty((n, 3), np.float64) for _ in range(3))
+        _check(self._lib.nb_batch_download(self._b, cid, _ptr(p), _ptr(v), _ptr(a)))
+        return p, v, a
+
+    def energy(self, cid: int):
+        ek = C.c_double()
+        ep = C.c_double()
+        _check(self._lib.nb_batch_energy(self._b, cid, C.byref(ek), C.byref(ep)))
+        return ek.value, ep.value
+
+    def info(self):
+        c = C.c_int32()
+        b = C.c_int64()
+        i = C.c_int64()
+        _check(self._lib.nb_batch_info(self._b, C.byref(c), C.byref(b), C.byref(i)))
+        return c.value, b.value, i.value
+
+    def set_timing(self, enable: bool):
+        _check(self._lib.nb_batch_set_timing(self._b, 1 if enable else 0))
+
+    def timing(self) -> dict:
+        t = Timing()
+        _check(self._lib.nb_batch_get_timing(self._b, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Timing._fields_}
