@@ -75,7 +75,7 @@ typedef struct nb_config {
   int32_t flags;      /* NB_F_* */
 } nb_config;
 
-#define NB_F_NO_GRAPH 0x1 /* do not capture multi-step nb_step calls into a hipGraph */
+#define NB_F_NO_GRAPH 0x1 /* launch steps eagerly instead of replaying a captured hipGraph */
 
 typedef struct nb_ctx nb_ctx;
 
@@ -195,6 +195,63 @@ int nb_batch_energy(nb_batch* b, int32_t id, double* ekin, double* epot);
 int nb_batch_info(nb_batch* b, int32_t* clusters, int64_t* bodies, int64_t* interactions);
 int nb_batch_set_timing(nb_batch* b, int32_t enable);
 int nb_batch_get_timing(nb_batch* b, nb_timing* out);
+
+/* ---- serving ---------------------------------------------------------------
+ * A request loop over many simulation jobs: jobs are submitted at any time,
+ * admitted in arrival order while their device memory fits the HBM budget (a
+ * bounded look-ahead lets small jobs pass a large one that does not fit yet),
+ * and every admitted job advances one step per nb_server_tick. Direct-sum jobs
+ * (n < bh_threshold) share one nb_batch launch; Barnes-Hut jobs run their own
+ * contexts on their own streams, so their kernels overlap on the device.
+ * Finished jobs release their memory and keep their FP64 final state on the
+ * host until fetched. A job's result is the same alone or in any mix. */
+typedef struct nb_server nb_server;
+typedef struct nb_server_config {
+  int64_t hbm_budget;    /* device bytes jobs may hold; 0 = 60% of free memory */
+  int64_t direct_pool;   /* bodies in the shared direct-sum pool; 0 = 2^24 */
+  int32_t max_jobs;      /* jobs running at once; 0 = 1024 */
+  int32_t bh_threshold;  /* n >= this runs Barnes-Hut; 0 = 65536 */
+  float theta;           /* Barnes-Hut opening angle; 0 = 0.5 */
+  int32_t leaf_size;     /* Barnes-Hut bodies per leaf; 0 = default */
+  int32_t ic_threads;    /* host threads preparing initial conditions; 0 = 4 */
+  int32_t lookahead;     /* queued jobs considered per admission pass; 0 = 16 */
+} nb_server_config;
+
+#define NB_IC_PLUMMER 0 /* a = 50, total mass 1e4, truncated at 10 a */
+#define NB_IC_BALL 1    /* uniform ball, R = 100, m = 1 */
+
+typedef struct nb_job {
+  int64_t n;
+  int32_t steps;
+  int32_t ic;            /* NB_IC_* */
+  uint64_t seed;
+  float eps2, dt, G;     /* G = 0 means 1 */
+  int32_t pad_;
+} nb_job;
+
+#define NB_JOB_QUEUED 0
+#define NB_JOB_RUNNING 1
+#define NB_JOB_DONE 2
+#define NB_JOB_FAILED (-1)
+
+typedef struct nb_job_status {
+  int32_t state;         /* NB_JOB_* */
+  int32_t solver;        /* nb_solver chosen for the job */
+  int32_t steps_done, steps;
+  double t_submit, t_admit, t_done;  /* seconds since nb_server_create */
+} nb_job_status;
+
+int nb_server_create(nb_server** out, const nb_server_config* cfg);
+void nb_server_destroy(nb_server* s);
+int nb_server_submit(nb_server* s, const nb_job* job, int64_t* id);
+/* Admit what fits, advance every running job one step, retire finished jobs.
+ * running/finished (optional) receive the running count and the jobs retired. */
+int nb_server_tick(nb_server* s, int32_t* running, int32_t* finished);
+int nb_server_status(nb_server* s, int64_t id, nb_job_status* out);
+/* FP64 final state of a finished job (double[3n] each, interleaved). */
+int nb_server_result(nb_server* s, int64_t id, double* pos3, double* vel3);
+int nb_server_forget(nb_server* s, int64_t id);
+double nb_server_now(nb_server* s);
 
 /* Library / device description for logs: writes a NUL-terminated string. */
 int nb_device_info(char* buf, int32_t cap);

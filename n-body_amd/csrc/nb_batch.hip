@@ -482,7 +482,7 @@ int nb_batch_step(nb_batch* b, int32_t nsteps) {
     }
     return NB_OK;
   }
-  if (nsteps == 1 || (b->cfg.flags & NB_F_NO_GRAPH)) {
+  if (b->cfg.flags & NB_F_NO_GRAPH) {
     for (int s = 0; s < nsteps; ++s) NB_HIP(enqueue_step(b));
     return NB_OK;
   }

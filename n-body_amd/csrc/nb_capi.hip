@@ -256,7 +256,9 @@ int nb_step(nb_ctx* c, int32_t nsteps) {
     c->tree_valid = c->cfg.solver == NB_SOLVER_BH;
     return NB_OK;
   }
-  const bool use_graph = nsteps > 1 && !(c->cfg.flags & NB_F_NO_GRAPH);
+  // The step is captured once per context and replayed (cheap launches for the
+  // ~80-kernel Barnes-Hut step); graph and eager launches give identical results.
+  const bool use_graph = !(c->cfg.flags & NB_F_NO_GRAPH);
   if (!use_graph) {
     for (int s = 0; s < nsteps; ++s) NB_HIP(nb_enqueue_step(c));
   } else {

@@ -109,6 +109,48 @@ class BatchConfig(C.Structure):
     ]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [
+        ("hbm_budget", C.c_int64),
+        ("direct_pool", C.c_int64),
+        ("max_jobs", C.c_int32),
+        ("bh_threshold", C.c_int32),
+        ("theta", C.c_float),
+        ("leaf_size", C.c_int32),
+        ("ic_threads", C.c_int32),
+        ("lookahead", C.c_int32),
+    ]
+
+
+class Job(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("steps", C.c_int32),
+        ("ic", C.c_int32),
+        ("seed", C.c_uint64),
+        ("eps2", C.c_float),
+        ("dt", C.c_float),
+        ("G", C.c_float),
+        ("pad_", C.c_int32),
+    ]
+
+
+class JobStatus(C.Structure):
+    _fields_ = [
+        ("state", C.c_int32),
+        ("solver", C.c_int32),
+        ("steps_done", C.c_int32),
+        ("steps", C.c_int32),
+        ("t_submit", C.c_double),
+        ("t_admit", C.c_double),
+        ("t_done", C.c_double),
+    ]
+
+
+IC_PLUMMER = 0
+IC_BALL = 1
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, -1
+
 _lib = None
 
 _P = C.c_void_p
@@ -146,6 +188,14 @@ _SIGS = {
     "nb_batch_info": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
     "nb_batch_set_timing": (C.c_int, [_P, C.c_int32]),
     "nb_batch_get_timing": (C.c_int, [_P, C.POINTER(Timing)]),
+    "nb_server_create": (C.c_int, [C.POINTER(_P), C.POINTER(ServerConfig)]),
+    "nb_server_destroy": (None, [_P]),
+    "nb_server_submit": (C.c_int, [_P, C.POINTER(Job), C.POINTER(C.c_int64)]),
+    "nb_server_tick": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "nb_server_status": (C.c_int, [_P, C.c_int64, C.POINTER(JobStatus)]),
+    "nb_server_result": (C.c_int, [_P, C.c_int64, _P, _P]),
+    "nb_server_forget": (C.c_int, [_P, C.c_int64]),
+    "nb_server_now": (C.c_double, [_P]),
     "nb_ic_ball": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, C.c_double, _P, _P, _P]),
     "nb_ic_plummer": (
         C.c_int,
@@ -488,3 +538,67 @@ class Batch:
         t = Timing()
         _check(self._lib.nb_batch_get_timing(self._b, C.byref(t)))
         return {k: getattr(t, k) for k, _ in Timing._fields_}
+
+
+class Server:
+    """Continuous-batching request loop (nb_server_*)."""
+
+    def __init__(self, hbm_budget: int = 0, direct_pool: int = 0, max_jobs: int = 0,
+                 bh_threshold: int = 0, theta: float = 0.0, leaf_size: int = 0, ic_threads: int = 0,
+                 lookahead: int = 0):
+        self._lib = load()
+        cfg = ServerConfig(hbm_budget=hbm_budget, direct_pool=direct_pool, max_jobs=max_jobs,
+                           bh_threshold=bh_threshold, theta=theta, leaf_size=leaf_size,
+                           ic_threads=ic_threads, lookahead=lookahead)
+        self._s = C.c_void_p()
+        _check(self._lib.nb_server_create(C.byref(self._s), C.byref(cfg)))
+        self.sizes: dict[int, int] = {}
+
+    def close(self):
+        if self._s:
+            self._lib.nb_server_destroy(self._s)
+            self._s = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, n: int, steps: int, ic: int = IC_PLUMMER, seed: int = 0, eps2: float = 40.0,
+               dt: float = 0.0035, G: float = 1.0) -> int:
+        job = Job(n=n, steps=steps, ic=ic, seed=seed, eps2=eps2, dt=dt, G=G)
+        jid = C.c_int64()
+        _check(self._lib.nb_server_submit(self._s, C.byref(job), C.byref(jid)))
+        self.sizes[jid.value] = n
+        return jid.value
+
+    def tick(self):
+        r = C.c_int32()
+        f = C.c_int32()
+        _check(self._lib.nb_server_tick(self._s, C.byref(r), C.byref(f)))
+        return r.value, f.value
+
+    def status(self, jid: int) -> dict:
+        st = JobStatus()
+        _check(self._lib.nb_server_status(self._s, jid, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in JobStatus._fields_}
+
+    def result(self, jid: int):
+        n = self.sizes[jid]
+        p = np.empty((n, 3), np.float64)
+        v = np.empty((n, 3), np.float64)
+        _check(self._lib.nb_server_result(self._s, jid, _ptr(p), _ptr(v)))
+        return p, v
+
+    def forget(self, jid: int):
+        _check(self._lib.nb_server_forget(self._s, jid))
+
+    def now(self) -> float:
+        return self._lib.nb_server_now(self._s)
