@@ -13,7 +13,11 @@ Also reported on the same line:
   cpu_baseline  the oracle's v2 restatement (OpenMP, the reference's flags) on a
                 bounded sample of the same workload on this host;
   bh_4m         Barnes-Hut step time on two colliding exponential discs,
-                N = 2^22, theta = 0.5 (skip with --no-bh).
+                N = 2^22, theta = 0.5 (skip with --no-bh);
+  batch_c4      512 independent Plummer clusters (N_c log-uniform in
+                [256, 32768]) in one persistent launch per step (--no-batch);
+  serve         with --serve S: the Poisson job stream through the continuous-
+                batching server and one job at a time, S seconds each.
 """
 from __future__ import annotations
 
@@ -23,6 +27,8 @@ import os
 import sys
 import time
 from pathlib import Path
+
+import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "n-body_amd"))
@@ -63,8 +69,6 @@ def cpu_baseline(pos, mass, budget_s: float):
     """v2 (ParticleSolverCPUParallel.cpp:12-40, OpenMP static) force pass over a
     bounded set of targets against all N sources, on this host's cores."""
     sys.path.insert(0, str(ROOT / "oracle"))
-    import numpy as np
-
     import oracle as ora
 
     n = pos.shape[0]
@@ -96,8 +100,6 @@ def cpu_baseline(pos, mass, budget_s: float):
 
 
 def bench_bh(nb, steps: int, warmup: int):
-    import numpy as np
-
     n = 1 << 22
     t0 = time.perf_counter()
     pos, vel, m = nb.ic_discs(2024, n)
@@ -126,6 +128,127 @@ def bench_bh(nb, steps: int, warmup: int):
     }
 
 
+def c4_clusters(nb, count=512, seed=4000):
+    """512 Plummer clusters, N_c log-uniform in [256, 32768], eps2 ~ U[1, 40],
+    dt ~ U[0.001, 0.005]; each cluster a = 10, m = 0.1 per body."""
+    sizes = nb.log_uniform_sizes(seed, count, 256, 32768)
+    eps2 = nb.uniform(seed + 1, count, 1.0, 40.0)
+    dts = nb.uniform(seed + 2, count, 0.001, 0.005)
+    out = []
+    for c in range(count):
+        pos, vel, m = nb.ic_plummer(seed * 1000 + c, int(sizes[c]), 10.0, 0.1 * int(sizes[c]))
+        out.append((pos, vel, m, float(eps2[c]), float(dts[c])))
+    return out
+
+
+def bench_batch(nb, steps: int, warmup: int, energy: bool):
+    t0 = time.perf_counter()
+    cl = c4_clusters(nb)
+    log(f"batch: 512 clusters generated ({time.perf_counter() - t0:.1f} s)")
+    total = sum(len(c[2]) for c in cl)
+    with nb.Batch(sum((len(c[2]) + 255) // 256 * 256 for c in cl), len(cl)) as b:
+        ids = [b.add(p, v, m, e, d) for p, v, m, e, d in cl]
+        e0 = [b.energy(i) for i in ids] if energy else None
+        _, bodies, inter = b.info()
+        b.step(warmup)
+        b.sync()
+        b.set_timing(True)
+        t0 = time.perf_counter()
+        b.step(steps)
+        b.sync()
+        wall = time.perf_counter() - t0
+        t = b.timing()
+        res = {
+            "workload": "512 Plummer clusters, N_c log-uniform [256, 32768] (seeded), eps2~U[1,40], "
+                        "dt~U[0.001,0.005], one persistent force launch + one integrate launch per step",
+            "bodies": total,
+            "interactions_per_step": inter,
+            "steps": steps,
+            "ms_per_step": 1e3 * wall / steps,
+            "force_ms": t["force_ms"] / max(1, t["force_calls"]),
+            "interactions_per_s": inter * steps / wall,
+        }
+        if energy:
+            e1 = [b.energy(i) for i in ids]
+            drift = np.array([abs((a[0] + a[1]) - (c[0] + c[1])) / abs(c[0] + c[1])
+                              for a, c in zip(e1, e0)])
+            res["energy_drift_p50"] = float(np.median(drift))
+            res["energy_drift_max"] = float(drift.max())
+            res["energy_steps"] = steps + warmup
+    return res
+
+
+def poisson_jobs(seed: int, rate: float, duration: float):
+    """The serving stream: Poisson arrivals; N log-uniform in [1024, 1048576];
+    steps uniform in [10, 500]; Plummer or uniform ball with p = 1/2; eps2 = 40."""
+    rng = np.random.default_rng(seed)
+    t, jobs = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > duration:
+            break
+        n = int(np.floor(np.exp(rng.uniform(np.log(1024), np.log(1 << 20)))))
+        jobs.append({"t": t, "n": n, "steps": int(rng.integers(10, 501)), "ic": int(rng.integers(0, 2)),
+                     "seed": int(rng.integers(0, 2 ** 62))})
+    return jobs
+
+
+def run_stream(nb, jobs, limit_s: float, one_at_a_time: bool):
+    """Feed the job stream in real time; stop admitting work after limit_s.
+    Returns per-job records and totals."""
+    kw = dict(max_jobs=1) if one_at_a_time else {}
+    done, body_steps = [], 0
+    with nb.Server(**kw) as s:
+        ids, k = {}, 0
+        t_start = time.perf_counter()
+        while True:
+            now = time.perf_counter() - t_start
+            while k < len(jobs) and jobs[k]["t"] <= now:
+                j = jobs[k]
+                ids[s.submit(j["n"], j["steps"], j["ic"], j["seed"])] = j
+                k += 1
+            if now > limit_s:
+                break
+            running, finished = s.tick()
+            if finished:
+                for jid in list(ids):
+                    st = s.status(jid)
+                    if st["state"] == nb.JOB_DONE:
+                        done.append({"n": ids[jid]["n"], "steps": st["steps"],
+                                     "latency": st["t_done"] - st["t_submit"],
+                                     "solver": st["solver"]})
+                        body_steps += ids[jid]["n"] * st["steps"]
+                        s.forget(jid)
+                        del ids[jid]
+            elif running == 0 and k < len(jobs):
+                time.sleep(min(0.01, max(0.0, jobs[k]["t"] - now)))
+        queued = sum(1 for jid in ids if s.status(jid)["state"] == nb.JOB_QUEUED)
+        running = sum(1 for jid in ids if s.status(jid)["state"] == nb.JOB_RUNNING)
+    return done, body_steps, k, queued, running
+
+
+def bench_serve(nb, rate: float, duration: float, limit: float):
+    jobs = poisson_jobs(5050, rate, duration)
+    out = {"workload": f"Poisson stream {rate:g} jobs/s for {duration:g} s (seeded), N log-uniform "
+                       f"[1024, 2^20], direct below 65536 else Barnes-Hut theta=0.5, steps U[10,500], "
+                       f"Plummer or ball, eps2=40, dt=0.0035; both modes measured for {limit:g} s",
+           "offered_jobs": len(jobs)}
+    for mode, one in (("batched", False), ("one_at_a_time", True)):
+        log(f"serve: {mode}")
+        done, bs, submitted, queued, running = run_stream(nb, jobs, limit, one)
+        lat = np.array([d["latency"] for d in done]) if done else np.array([np.nan])
+        out[mode] = {"submitted": submitted, "completed": len(done), "running": running,
+                     "queued": queued, "body_steps_per_s": bs / limit,
+                     "completed_per_s": len(done) / limit,
+                     "latency_p50_s": float(np.percentile(lat, 50)),
+                     "latency_p99_s": float(np.percentile(lat, 99))}
+        log(f"serve: {mode}: {out[mode]}")
+    b, o = out["batched"], out["one_at_a_time"]
+    out["throughput_ratio_body_steps"] = b["body_steps_per_s"] / max(o["body_steps_per_s"], 1e-9)
+    out["throughput_ratio_jobs"] = b["completed_per_s"] / max(o["completed_per_s"], 1e-9)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,13 +261,17 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bh", action="store_true")
     ap.add_argument("--bh-steps", type=int, default=20)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-energy", action="store_true", help="per-cluster FP64 energy drift")
+    ap.add_argument("--serve", type=float, default=0.0,
+                    help="also run the serving stream for this many seconds per mode")
+    ap.add_argument("--serve-rate", type=float, default=50.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this engine has no multi-GPU path "
               "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
-
-    import numpy as np
 
     import nbody_amd as nb
 
@@ -204,6 +331,12 @@ def main():
         log("bh: N=2^22 two-disc collision")
         result["bh_4m"] = bench_bh(nb, args.bh_steps, 2)
         log(f"bh: {result['bh_4m']['ms_per_step']:.2f} ms/step")
+    if not args.no_batch:
+        log("batch: C4 ensemble")
+        result["batch_c4"] = bench_batch(nb, args.batch_steps, 2, args.batch_energy)
+        log(f"batch: {result['batch_c4']['ms_per_step']:.2f} ms/step")
+    if args.serve > 0:
+        result["serve"] = bench_serve(nb, args.serve_rate, args.serve, args.serve)
     if not args.no_cpu:
         log("cpu baseline (oracle v2 restatement)")
         result["cpu_baseline"] = cpu_baseline(pos.astype(np.float32), m, args.cpu_budget)

@@ -377,6 +377,279 @@ int or_v5_forces(int64_t n, const float* pos4, const float* mass4, float eps2, f
 }
 
 /* ------------------------------------------------------------------------ */
+/* v6 parallel octree (ParallelOctreeCPU.cpp:7-505), int32 links.            */
+/* ------------------------------------------------------------------------ */
+static double wall_s(void) {
+#ifdef _OPENMP
+  return omp_get_wtime();
+#else
+  return 0.0;
+#endif
+}
+
+/* Subtree insertion of ParallelOctreeCPU::insert(pos, mass, subTreeId)
+ * (ParallelOctreeCPU.cpp:359-414): the v5 loop with the node counter and parent
+ * list of the task, and the merge rule d^2 < 1e-3 or 10 extra subdivisions. */
+typedef struct v6_task {
+  int64_t base, cap, count; /* node pool [base, base + cap), nodes used */
+  int32_t* parents;
+  int64_t nparents;
+  int overflow;
+} v6_task;
+
+static void v6_subdivide(v5_node* nodes, v6_task* tk, int32_t i) {
+  if (tk->count + 8 > tk->cap) { tk->overflow = 1; return; }
+  const int32_t first = (int32_t)(tk->base + tk->count);
+  v5_node* nd = &nodes[i];
+  nd->first_child = first;
+  tk->parents[tk->nparents++] = i;
+  tk->count += 8;
+  const float w = nd->maxb[0] - nd->minb[0], h = nd->maxb[1] - nd->minb[1];
+  const float d = nd->maxb[2] - nd->minb[2];
+  for (int j = 0; j < 8; ++j) {
+    v5_node* c = &nodes[first + j];
+    if (j & 4) { c->minb[2] = nd->minb[2]; c->maxb[2] = nd->minb[2] + (d / 2.f); }
+    else       { c->minb[2] = nd->minb[2] + (d / 2.f); c->maxb[2] = nd->maxb[2]; }
+    if (j & 2) { c->minb[1] = nd->minb[1]; c->maxb[1] = nd->minb[1] + (h / 2.f); }
+    else       { c->minb[1] = nd->minb[1] + (h / 2.f); c->maxb[1] = nd->maxb[1]; }
+    if (j & 1) { c->minb[0] = nd->minb[0]; c->maxb[0] = nd->minb[0] + (w / 2.f); }
+    else       { c->minb[0] = nd->minb[0] + (w / 2.f); c->maxb[0] = nd->maxb[0]; }
+    c->mass = -1.f;
+    c->first_child = -1;
+    if (j < 7) c->next = first + j + 1;
+  }
+  nodes[first + 7].next = nd->next;
+}
+
+static int32_t v6_next(const v5_node* nodes, const float p[3], int32_t i) {
+  float c[3];
+  v5_center(&nodes[i], c);
+  return nodes[i].first_child + ((p[0] < c[0]) | ((p[1] < c[1]) << 1) | ((p[2] < c[2]) << 2));
+}
+
+static void v6_insert(v5_node* nodes, v6_task* tk, int32_t root, const float p[3], float m) {
+  int32_t i = root;
+  while (!tk->overflow) {
+    v5_node* nd = &nodes[i];
+    const int leaf = nd->first_child < 0, occ = nd->mass > 0.f;
+    if (leaf && !occ) { nd->com[0] = p[0]; nd->com[1] = p[1]; nd->com[2] = p[2]; nd->mass = m; return; }
+    if (leaf && occ) {
+      const float p2[3] = {nd->com[0], nd->com[1], nd->com[2]};
+      const float m2 = nd->mass;
+      v6_subdivide(nodes, tk, i);
+      if (tk->overflow) return;
+      int32_t c1 = v6_next(nodes, p, i), c2 = v6_next(nodes, p2, i);
+      const float dx = p2[0] - p[0], dy = p2[1] - p[1], dz = p2[2] - p[2];
+      const float dist = (dx * dx + dy * dy) + (dz * dz + 0.f);
+      int count = 0;
+      while (c1 == c2) {
+        if (dist < 1e-3 || count++ == 10) {
+          v5_node* c = &nodes[c1];
+          c->com[0] = p[0]; c->com[1] = p[1]; c->com[2] = p[2];
+          c->mass = m2 + m;
+          return;
+        }
+        v6_subdivide(nodes, tk, c1);
+        if (tk->overflow) return;
+        c1 = v6_next(nodes, p, c1);
+        c2 = v6_next(nodes, p2, c2);
+      }
+      v5_node* a = &nodes[c1];
+      a->com[0] = p[0]; a->com[1] = p[1]; a->com[2] = p[2]; a->mass = m;
+      v5_node* b = &nodes[c2];
+      b->com[0] = p2[0]; b->com[1] = p2[1]; b->com[2] = p2[2]; b->mass = m2;
+      return;
+    }
+    i = v6_next(nodes, p, i);
+  }
+}
+
+static void v6_propagate_list(v5_node* nodes, const int32_t* parents, int64_t k0, int64_t k1) {
+  for (int64_t k = k1 - 1; k >= k0; --k) {
+    v5_node* par = &nodes[parents[k]];
+    float c[3] = {0.f, 0.f, 0.f}, m = 0.f;
+    for (int j = 0; j < 8; ++j) {
+      const v5_node* ch = &nodes[par->first_child + j];
+      if (!(ch->mass > 0.f)) continue;
+      c[0] += ch->com[0] * ch->mass;
+      c[1] += ch->com[1] * ch->mass;
+      c[2] += ch->com[2] * ch->mass;
+      m += ch->mass;
+    }
+    par->com[0] = c[0] / m; par->com[1] = c[1] / m; par->com[2] = c[2] / m;
+    par->mass = m;
+  }
+}
+
+static void v6_prune_list(v5_node* nodes, const int32_t* parents, int64_t k0, int64_t k1) {
+  for (int64_t k = k0; k < k1; ++k) {
+    v5_node* par = &nodes[parents[k]];
+    if (!(par->mass > 0.f)) { par->first_child = par->next; continue; }
+    int32_t first = -1, last = -1;
+    for (int j = 0; j < 8; ++j) {
+      const int32_t ci = par->first_child + j;
+      if (nodes[ci].mass > 0.f) {
+        if (first == -1) { first = last = ci; continue; }
+        nodes[last].next = ci;
+        last = ci;
+      }
+    }
+    nodes[last].next = par->next;
+    par->first_child = first;
+  }
+}
+
+int or_v6_forces(int64_t n, const float* pos4, const float* mass4, float eps2, float G,
+                 float theta, float* force4, int64_t t0, int64_t t1, int nthreads,
+                 double* build_s, double* force_s) {
+  nthreads = clamp_threads(nthreads);
+  const double tb = wall_s();
+  /* ParallelOctreeCPU ctor (:7-56): top tree of depth <= 3 */
+  int64_t top = 0, width = 1;
+  int depth = -1;
+  while (top < n && depth < 3) {
+    if (top + width >= n) break;
+    top += width;
+    width *= 8;
+    depth++;
+  }
+  if (depth < 0) { top = 1; depth = 0; width = 1; }
+  const int64_t ntasks = (int64_t)1 << (3 * depth);
+  /* bucket counts per task (deepest top cells), then per-task pools sized by need */
+  int64_t* tcount = (int64_t*)calloc((size_t)ntasks, sizeof(int64_t));
+  int64_t* tbase = (int64_t*)calloc((size_t)ntasks + 1, sizeof(int64_t));
+  int32_t* troot = (int32_t*)malloc(sizeof(int32_t) * (size_t)ntasks);
+  for (int64_t t = 0; t < ntasks; ++t) troot[t] = -1;
+  /* the top tree is built first so cells can be found */
+  const int64_t top_nodes = top;
+  v5_node* topn = (v5_node*)calloc((size_t)top_nodes, sizeof(v5_node));
+  float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  float mn0 = FLT_MAX, mn1 = FLT_MAX, mn2 = FLT_MAX, mx0 = -FLT_MAX, mx1 = -FLT_MAX, mx2 = -FLT_MAX;
+#pragma omp parallel for num_threads(nthreads) reduction(min : mn0, mn1, mn2) reduction(max : mx0, mx1, mx2)
+  for (int64_t i = 0; i < n; ++i) {
+    const float* p = &pos4[4 * i];
+    if (p[0] < mn0) mn0 = p[0];
+    if (p[1] < mn1) mn1 = p[1];
+    if (p[2] < mn2) mn2 = p[2];
+    if (p[0] > mx0) mx0 = p[0];
+    if (p[1] > mx1) mx1 = p[1];
+    if (p[2] > mx2) mx2 = p[2];
+  }
+  mn[0] = mn0; mn[1] = mn1; mn[2] = mn2; mx[0] = mx0; mx[1] = mx1; mx[2] = mx2;
+  /* complete top tree: subdivide nodes 0, 1, 2, ... until top_nodes exist (:127-131) */
+  v5_node* root = &topn[0];
+  for (int a = 0; a < 3; ++a) { root->minb[a] = mn[a]; root->maxb[a] = mx[a]; }
+  root->first_child = -1;
+  root->mass = -1.f;
+  root->next = -1;
+  int32_t* top_parents = (int32_t*)malloc(sizeof(int32_t) * (size_t)(top_nodes + ntasks));
+  int64_t ntop_parents = 0;
+  {
+    v6_task tk = {0, top_nodes, 1, top_parents, 0, 0};
+    for (int32_t r = 0; tk.count < top_nodes; ++r) v6_subdivide(topn, &tk, r);
+    ntop_parents = tk.nparents;
+  }
+  /* fill the tasks under one lock per deepest cell (:134-152) */
+  int32_t* cell_of = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
+#pragma omp parallel for num_threads(nthreads)
+  for (int64_t j = 0; j < n; ++j) {
+    int32_t i = 0;
+    for (int d = 0; d < depth; ++d) i = v6_next(topn, &pos4[4 * j], i);
+    cell_of[j] = i;
+  }
+  omp_lock_t* locks = (omp_lock_t*)malloc(sizeof(omp_lock_t) * (size_t)ntasks);
+  for (int64_t t = 0; t < ntasks; ++t) omp_init_lock(&locks[t]);
+  for (int64_t j = 0; j < n; ++j) tcount[cell_of[j] % ntasks]++;
+  for (int64_t t = 0; t < ntasks; ++t) tbase[t + 1] = tbase[t] + tcount[t];
+  int32_t* tparts = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
+  int64_t* tfill = (int64_t*)calloc((size_t)ntasks, sizeof(int64_t));
+#pragma omp parallel for num_threads(nthreads)
+  for (int64_t j = 0; j < n; ++j) {
+    const int32_t i = cell_of[j];
+    const int64_t t = i % ntasks;
+    omp_set_lock(&locks[t]);
+    tparts[tbase[t] + tfill[t]++] = (int32_t)j;
+    troot[t] = i;
+    omp_unset_lock(&locks[t]);
+  }
+  /* node pools: top tree first, then one pool per task (16 nodes per body + 64) */
+  int64_t* pbase = (int64_t*)malloc(sizeof(int64_t) * (size_t)(ntasks + 1));
+  pbase[0] = top_nodes;
+  for (int64_t t = 0; t < ntasks; ++t) pbase[t + 1] = pbase[t] + 16 * tcount[t] + 64;
+  const int64_t total_nodes = pbase[ntasks];
+  v5_node* nodes = (v5_node*)malloc(sizeof(v5_node) * (size_t)total_nodes);
+  int32_t* sparents = (int32_t*)malloc(sizeof(int32_t) * (size_t)total_nodes);
+  memcpy(nodes, topn, sizeof(v5_node) * (size_t)top_nodes);
+  v6_task* tasks = (v6_task*)calloc((size_t)ntasks, sizeof(v6_task));
+  int overflow = 0;
+  /* one subtree per task, schedule(dynamic) (:158-189) */
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads) reduction(| : overflow)
+  for (int64_t t = 0; t < ntasks; ++t) {
+    if (troot[t] < 0) continue;
+    v6_task* tk = &tasks[t];
+    tk->base = pbase[t];
+    tk->cap = pbase[t + 1] - pbase[t];
+    tk->count = 1;
+    tk->parents = &sparents[pbase[t]];
+    const int32_t r = (int32_t)pbase[t];
+    v5_node* oct = &nodes[r];
+    for (int a = 0; a < 3; ++a) { oct->minb[a] = nodes[troot[t]].minb[a]; oct->maxb[a] = nodes[troot[t]].maxb[a]; }
+    oct->first_child = -1;
+    oct->mass = -1.f;
+    oct->next = nodes[troot[t]].next;
+    nodes[troot[t]].first_child = r;
+    for (int64_t k = 0; k < tcount[t]; ++k) {
+      const int32_t pid = tparts[tbase[t] + k];
+      v6_insert(nodes, tk, r, &pos4[4 * pid], mass4[4 * pid]);
+    }
+    v6_propagate_list(nodes, tk->parents, 0, tk->nparents);
+    overflow |= tk->overflow;
+  }
+  /* top-level propagate (:262-291) and prune (:208-245) */
+  for (int64_t c = top_nodes - ntasks; c < top_nodes; ++c)
+    if (tcount[c % ntasks] > 0) top_parents[ntop_parents++] = (int32_t)c;
+#pragma omp parallel for num_threads(nthreads)
+  for (int64_t k = ntop_parents - 1; k >= top_nodes - ntasks; --k) {
+    const int32_t pi = top_parents[k];
+    const v5_node* ch = &nodes[nodes[pi].first_child];
+    if (!(ch->mass > 0.f)) continue;
+    nodes[pi].com[0] = ch->com[0]; nodes[pi].com[1] = ch->com[1]; nodes[pi].com[2] = ch->com[2];
+    nodes[pi].mass = ch->mass;
+  }
+  v6_propagate_list(nodes, top_parents, 0, top_nodes - ntasks);
+  v6_prune_list(nodes, top_parents, 0, top_nodes - ntasks);
+#pragma omp parallel for num_threads(nthreads)
+  for (int64_t c = top_nodes - ntasks; c < top_nodes; ++c)
+    if (nodes[c].mass > 0.f) nodes[nodes[c].first_child].next = nodes[c].next;
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+  for (int64_t t = 0; t < ntasks; ++t)
+    if (tcount[t] > 0 && troot[t] >= 0) v6_prune_list(nodes, tasks[t].parents, 0, tasks[t].nparents);
+  const double tf = wall_s();
+  if (!overflow) {
+    v5_tree view;
+    view.nodes = nodes;
+    const float theta2 = theta * theta;
+    /* force loop, schedule(dynamic) (ParticleSolverBHutCPUParallel.cpp:28-31) */
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+    for (int64_t i = t0; i < t1; ++i) {
+      float f[3];
+      v5_force(&view, &pos4[4 * i], eps2, G, theta2, f);
+      force4[4 * i] = f[0];
+      force4[4 * i + 1] = f[1];
+      force4[4 * i + 2] = f[2];
+      force4[4 * i + 3] = 0.f;
+    }
+  }
+  const double te = wall_s();
+  if (build_s) *build_s = tf - tb;
+  if (force_s) *force_s = te - tf;
+  for (int64_t t = 0; t < ntasks; ++t) omp_destroy_lock(&locks[t]);
+  free(locks); free(tcount); free(tbase); free(troot); free(topn); free(top_parents);
+  free(cell_of); free(tparts); free(tfill); free(pbase); free(nodes); free(sparents); free(tasks);
+  return overflow ? -5 : 0;
+}
+
+/* ------------------------------------------------------------------------ */
 /* CPU build of the engine's Morton / sort / octree.                         */
 /* ------------------------------------------------------------------------ */
 void or_bbox(int64_t n, const float* pos4, float bmin[3], float bmax[3]) {

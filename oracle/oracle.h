@@ -70,6 +70,20 @@ int or_v5_forces(int64_t n, const float* pos4, const float* mass4, float eps2, f
                  float theta, float* force4, double pool_factor, int64_t* node_count,
                  int nthreads);
 
+/* ---- v6 parallel Barnes-Hut restatement (ParallelOctreeCPU.cpp:7-505,
+ * ParticleSolverBHutCPUParallel.cpp:18-41), the timed CPU baseline ----
+ * Top tree of depth <= 3 (at most 512 tasks), bodies bucketed into the deepest
+ * top cells under one omp_lock per cell, one sequential subtree per task built
+ * with `schedule(dynamic)`, merge rule d^2 < 1e-3 or 10 extra levels, top-level
+ * COM, prune, then the force loop with `schedule(dynamic)` for targets [t0, t1).
+ * Deviations: int32 node links (the reference's float links corrupt trees
+ * above ~65.6k bodies, D3) and subtree pools sized by need (16 nodes per body
+ * of the task plus 64) instead of n/2 nodes per task (48 GiB at n = 2^22).
+ * Returns 0 or -5 (pool overflow); build_s/force_s receive the phase times. */
+int or_v6_forces(int64_t n, const float* pos4, const float* mass4, float eps2, float G,
+                 float theta, float* force4, int64_t t0, int64_t t1, int nthreads,
+                 double* build_s, double* force_s);
+
 /* ---- CPU build of the engine's Morton / sort / octree (topology reference) ---- */
 /* Exact min/max of the FP32 positions; max starts at -FLT_MAX. */
 void or_bbox(int64_t n, const float* pos4, float bmin[3], float bmax[3]);

@@ -26,6 +26,8 @@ _SIGS = {
     "or_euler_update_f64": (None, [_I, _P, _P, _P, _P, C.c_double]),
     "or_energy_f64": (None, [_I, _P, _P, _P, C.c_double, C.c_double, _P, _P, _P, C.c_int]),
     "or_v5_forces": (C.c_int, [_I, _P, _P, C.c_float, C.c_float, C.c_float, _P, C.c_double, _P, C.c_int]),
+    "or_v6_forces": (C.c_int, [_I, _P, _P, C.c_float, C.c_float, C.c_float, _P, _I, _I, C.c_int,
+                               C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "or_bbox": (None, [_I, _P, _P, _P]),
     "or_expand_bits": (C.c_uint32, [C.c_uint32]),
     "or_morton": (None, [_I, _P, _P, _P, _P]),
@@ -129,6 +131,20 @@ def v5_forces(pos4, mass4, eps2, theta=0.5, G=1.0, pool_factor=0.0, nthreads=Non
     rc = load(fast).or_v5_forces(n, _p(pos4), _p(mass4), eps2, G, theta, _p(out), pool_factor,
                                  C.byref(cnt), nthreads or threads())
     return rc, out, cnt.value
+
+
+def v6_forces(pos4, mass4, eps2, theta=0.5, G=1.0, t0=0, t1=None, nthreads=None, fast=False):
+    """v6 restatement: returns (rc, force4, build seconds, force seconds)."""
+    pos4 = np.ascontiguousarray(pos4, np.float32)
+    mass4 = np.ascontiguousarray(mass4, np.float32)
+    n = pos4.shape[0]
+    t1 = n if t1 is None else t1
+    out = np.zeros((n, 4), np.float32)
+    b = C.c_double()
+    f = C.c_double()
+    rc = load(fast).or_v6_forces(n, _p(pos4), _p(mass4), eps2, G, theta, _p(out), t0, t1,
+                                 nthreads or threads(), C.byref(b), C.byref(f))
+    return rc, out, b.value, f.value
 
 
 def bbox(pos4):

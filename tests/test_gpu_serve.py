@@ -63,7 +63,7 @@ def test_budget_limits_concurrency(nb):
     # 20000 bodies: Barnes-Hut jobs run one after another, all finish
     jobs = [(20000, 2, 0, 11), (20000, 2, 0, 12), (20000, 2, 0, 13)]
     pool = 1 << 16
-    with nb.Server(bh_threshold=8192, direct_pool=pool, hbm_budget=pool * 116 + 40 << 20) as s:
+    with nb.Server(bh_threshold=8192, direct_pool=pool, hbm_budget=pool * 116 + (18 << 20)) as s:
         ids = [s.submit(*j) for j in jobs]
         max_running = 0
         for _ in range(1000):

@@ -229,3 +229,28 @@ def test_v5_monopole_and_overflow(ora):
     # a pool of one node per body overflows: reported, not silent (reference D5)
     rc, _, _ = ora.v5_forces(pos, mass4(np.ones(16)), 0.01, theta=0.5, pool_factor=1.0)
     assert rc == -5
+
+
+def test_v6_matches_v5(ora):
+    # same geometric subdivision, opening rule and child order: the parallel
+    # restatement's forces equal the sequential one's (no close pairs to merge)
+    pos = vec4(seeded_ball(5000, seed=20))
+    mm = mass4(np.random.default_rng(21).uniform(0.5, 2.0, 5000))
+    rc5, f5, _ = ora.v5_forces(pos, mm, 40.0, theta=0.5)
+    rc6, f6, tb, tf = ora.v6_forces(pos, mm, 40.0, theta=0.5, nthreads=4)
+    assert rc5 == 0 and rc6 == 0
+    assert rel_err(f6, f5).max() < 1e-6
+    # a target range computes only those bodies
+    _, part, _, _ = ora.v6_forces(pos, mm, 40.0, theta=0.5, t0=100, t1=200, nthreads=2)
+    np.testing.assert_array_equal(part[100:200], f6[100:200])
+    assert np.all(part[:100] == 0)
+
+
+def test_v6_small_inputs(ora):
+    for n in (1, 2, 9, 70):
+        pos = vec4(seeded_ball(n, seed=n))
+        rc, f, _, _ = ora.v6_forces(pos, mass4(np.ones(n)), 1.0, theta=0.5, nthreads=2)
+        ref = ora.direct_f64(pos[:, :3].astype(np.float64), np.ones(n), 1.0)
+        assert rc == 0 and np.all(np.isfinite(f))
+        if n <= 9:
+            assert rel_err(f, ref).max() < 2e-2
