@@ -249,6 +249,24 @@ def bench_serve(nb, rate: float, duration: float, limit: float):
     return out
 
 
+def copy_probe(nbytes: int):
+    """One device-to-device copy of nbytes (read nbytes + write nbytes): the
+    reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE readings."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy probe: 3 copies of {nbytes} bytes")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -267,16 +285,30 @@ def main():
     ap.add_argument("--serve", type=float, default=0.0,
                     help="also run the serving stream for this many seconds per mode")
     ap.add_argument("--serve-rate", type=float, default=50.0)
+    ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (built-in shapes)")
+    ap.add_argument("--copy-probe", type=float, default=0.0,
+                    help="only run a device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this engine has no multi-GPU path "
               "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_probe > 0:
+        copy_probe(int(args.copy_probe * (1 << 20)))
+        return
+
     import nbody_amd as nb
 
     log(nb.device_info())
     n = args.n
+    tuning = None
+    if not args.no_tune:
+        t0 = time.perf_counter()
+        tuning = nb.autotune(n, nb.TUNE_DIRECT, 3)
+        if not args.no_bh:
+            tuning["bh_4m"] = nb.autotune(1 << 22, nb.TUNE_BH, 2)
+        log(f"autotune ({time.perf_counter() - t0:.1f} s): {tuning}")
     pos, vel, m = nb.ic_plummer(2018, n, 50.0, 1e4)
     with nb.Simulation(n, EPS2, DT, bodies_per_lane=args.bodies_per_lane,
                        j_split=args.j_split) as sim:
@@ -316,6 +348,8 @@ def main():
             "dt": DT,
             "force_ms": 1e3 * force_s,
             "pct_fp32_peak": 100.0 * achieved / FP32_PEAK_TFLOPS,
+            "shape": nb.get_tuning(n),
+            "autotune": tuning,
         },
         "roofline": {
             "bound": "valu",

@@ -253,6 +253,29 @@ int nb_server_result(nb_server* s, int64_t id, double* pos3, double* vel3);
 int nb_server_forget(nb_server* s, int64_t id);
 double nb_server_now(nb_server* s);
 
+/* ---- autotuning ---------------------------------------------------------
+ * Times candidate launch shapes on the current device with a seeded Plummer
+ * sphere of n bodies and installs the fastest as the default of later
+ * nb_create / nb_batch_create calls for systems in the same power-of-two size
+ * class (explicit non-zero config fields still win). Candidates: direct sum
+ * bodies_per_lane x j_split in {1,2,4} x {1,2,4}; Barnes-Hut leaf_size in
+ * {1,2,4,8,16,32}. The reference hard-codes its work-group sizes (64, 320,
+ * 1024) in the shaders. */
+typedef struct nb_tuning {
+  int64_t n;
+  int32_t bodies_per_lane;
+  int32_t j_split;
+  int32_t leaf_size;
+  int32_t pad_;
+  double direct_ms;  /* force pass at the chosen shape (0 if not tuned) */
+  double bh_ms;      /* Barnes-Hut build + traversal at the chosen leaf size */
+} nb_tuning;
+#define NB_TUNE_DIRECT 0x1
+#define NB_TUNE_BH 0x2
+int nb_autotune(int64_t n, int32_t what, int32_t reps, nb_tuning* out);
+/* The defaults nb_create would use for n bodies (tuned or built in). */
+int nb_get_tuning(int64_t n, nb_tuning* out);
+
 /* Library / device description for logs: writes a NUL-terminated string. */
 int nb_device_info(char* buf, int32_t cap);
 

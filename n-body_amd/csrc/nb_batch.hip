@@ -323,6 +323,7 @@ int nb_batch_create(nb_batch** out, const nb_batch_config* cfg) {
   if (!b) return nb_fail(NB_E_OOM, "nb_batch_create: host allocation failed");
   b->cfg = *cfg;
   b->cap = (cfg->max_bodies + 255) / 256 * 256;
+  // clusters are mostly small: one body per lane, four waves per work item
   b->bpl = cfg->bodies_per_lane > 0 ? cfg->bodies_per_lane : 1;
   b->jsplit = cfg->j_split > 0 ? cfg->j_split : 4;
   int dev = 0, cus = 256;

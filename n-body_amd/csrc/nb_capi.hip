@@ -62,13 +62,26 @@ int check_flags(nb_ctx* c) {
   return NB_OK;
 }
 
-// Default direct-sum launch shape; tuned on MI355X at N = 2^18 (bench.py --tune).
+}  // namespace
+
+// Built-in launch shapes (measured on MI355X): small systems use one body per
+// lane and four waves per block so that enough waves fill the 256 CUs.
+void nb_default_shape(int64_t n, int* bpl, int* jsplit, int* leaf) {
+  const int64_t np = (n + kPadBodies - 1) / kPadBodies * kPadBodies;
+  if (bpl && *bpl == 0) *bpl = np < 65536 ? 1 : 2;
+  if (jsplit && *jsplit == 0) *jsplit = np < 65536 ? 4 : 2;
+  if (leaf && *leaf == 0) *leaf = 8;
+}
+
+namespace {
+
 void default_direct_shape(nb_ctx* c) {
-  c->bpl = c->cfg.bodies_per_lane > 0 ? c->cfg.bodies_per_lane : 2;
-  c->jsplit = c->cfg.j_split > 0 ? c->cfg.j_split : 2;
-  // Small systems: more waves per body block keeps the 256 CUs busy.
-  if (c->cfg.bodies_per_lane <= 0 && c->n_pad < 65536) c->bpl = 1;
-  if (c->cfg.j_split <= 0 && c->n_pad < 65536) c->jsplit = 4;
+  int bpl = c->cfg.bodies_per_lane, js = c->cfg.j_split, leaf = c->cfg.leaf_size;
+  nb_tuned_shape(c->cfg.n, &bpl, &js, &leaf);
+  nb_default_shape(c->cfg.n, &bpl, &js, &leaf);
+  c->bpl = bpl;
+  c->jsplit = js;
+  c->cfg.leaf_size = leaf;
 }
 
 }  // namespace
@@ -149,7 +162,6 @@ int nb_create(nb_ctx** out, const nb_config* cfg) {
   const float theta = cfg->theta == 0.f ? 0.5f : cfg->theta;
   c->cfg.theta = theta;
   c->theta2 = theta * theta;
-  if (c->cfg.leaf_size == 0) c->cfg.leaf_size = 8;
   // The self pair gives dx = 0; its term is exactly 0 unless eps2 is so small
   // that (eps2)^{-3/2} overflows FP32. Below that, use the guarded kernel.
   c->guard = !(cfg->eps2 >= 1e-20f);

@@ -106,5 +106,10 @@ void nb_bh_free(nb_ctx* c);
 hipError_t nb_bh_build(nb_ctx* c);     // bbox, Morton, sort, permute, prefix, octree
 hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions);  // -> force
 
+// Launch-shape defaults: fill the zero fields from the autotuner's table, then
+// from the built-in defaults.
+bool nb_tuned_shape(int64_t n, int* bpl, int* jsplit, int* leaf);
+void nb_default_shape(int64_t n, int* bpl, int* jsplit, int* leaf);
+
 // step = forces + integration (no sync)
 hipError_t nb_enqueue_step(nb_ctx* c);

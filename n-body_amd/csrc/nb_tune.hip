@@ -1,0 +1,161 @@
+// nb_tune.hip — the autotuner of nbody.h (nb_autotune / nb_get_tuning).
+//
+// Candidates are timed with HIP events on a context's own stream; the winners
+// go into a process-wide table indexed by the power-of-two size class of n,
+// which nb_create consults when a config leaves the shape fields at 0.
+#include <cfloat>
+#include <mutex>
+#include <vector>
+
+#include "nb_internal.h"
+#include "nbody_ic.h"
+
+namespace {
+
+constexpr int kClasses = 31;
+std::mutex g_mu;
+nb_tuning g_table[kClasses] = {};
+
+int size_class(int64_t n) {
+  int c = 0;
+  while (c < kClasses - 1 && (int64_t(1) << c) < n) ++c;
+  return c;
+}
+
+struct Input {
+  std::vector<double> pos, vel;
+  std::vector<float> mass;
+};
+
+Input plummer(int64_t n) {
+  Input in;
+  in.pos.resize(3 * n);
+  in.vel.resize(3 * n);
+  in.mass.resize(n);
+  nb_ic_plummer(20240, n, 50.0, 1e4, 10.0, 1.0, in.pos.data(), in.vel.data(), in.mass.data());
+  return in;
+}
+
+// Mean device time of `reps` force passes (after one warm-up pass).
+int time_forces(const nb_config& cfg, const Input& in, int reps, double* ms) {
+  nb_ctx* c = nullptr;
+  int rc = nb_create(&c, &cfg);
+  if (rc) return rc;
+  rc = nb_upload_f64(c, in.pos.data(), in.vel.data(), nullptr, in.mass.data(), cfg.n);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = nb_fail(NB_E_HIP, "nb_autotune: event creation failed");
+  for (int r = -1; !rc && r < reps; ++r) {
+    if (r == 0) (void)hipEventRecord(e0, c->stream);
+    hipError_t e;
+    if (cfg.solver == NB_SOLVER_BH) {
+      e = nb_bh_build(c);
+      if (e == hipSuccess) e = nb_bh_traverse(c, false);
+    } else {
+      e = nb_launch_direct(c);
+    }
+    if (e != hipSuccess) rc = nb_fail(NB_E_HIP, std::string("nb_autotune: ") + hipGetErrorString(e));
+  }
+  if (!rc) {
+    (void)hipEventRecord(e1, c->stream);
+    float t = 0.f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess)
+      rc = nb_fail(NB_E_HIP, "nb_autotune: timing failed");
+    *ms = t / reps;
+  }
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  nb_destroy(c);
+  return rc;
+}
+
+}  // namespace
+
+// Consulted by nb_create / nb_batch_create: fills zero shape fields of the
+// class of n from the table; returns false if nothing was tuned for it.
+bool nb_tuned_shape(int64_t n, int* bpl, int* jsplit, int* leaf) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  const nb_tuning& t = g_table[size_class(n)];
+  if (t.n == 0) return false;
+  if (bpl && *bpl == 0) *bpl = t.bodies_per_lane;
+  if (jsplit && *jsplit == 0) *jsplit = t.j_split;
+  if (leaf && *leaf == 0) *leaf = t.leaf_size;
+  return true;
+}
+
+extern "C" int nb_autotune(int64_t n, int32_t what, int32_t reps, nb_tuning* out) {
+  if (n < 1 || n >= (int64_t(1) << 27)) return nb_fail(NB_E_ARG, "nb_autotune: n out of range");
+  if (reps < 1) reps = 3;
+  const Input in = plummer(n);
+  nb_tuning best{};
+  best.n = n;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    const nb_tuning& t = g_table[size_class(n)];
+    best.leaf_size = t.leaf_size;
+    best.bodies_per_lane = t.bodies_per_lane;
+    best.j_split = t.j_split;
+  }
+  if (what & NB_TUNE_DIRECT) {
+    double best_ms = DBL_MAX;
+    for (int b : {1, 2, 4})
+      for (int k : {1, 2, 4}) {
+        nb_config cfg{};
+        cfg.n = n;
+        cfg.eps2 = 40.f;
+        cfg.dt = 0.0035f;
+        cfg.solver = NB_SOLVER_DIRECT;
+        cfg.bodies_per_lane = b;
+        cfg.j_split = k;
+        double ms = 0;
+        const int rc = time_forces(cfg, in, reps, &ms);
+        if (rc) return rc;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best.bodies_per_lane = b;
+          best.j_split = k;
+          best.direct_ms = ms;
+        }
+      }
+  }
+  if (what & NB_TUNE_BH) {
+    double best_ms = DBL_MAX;
+    for (int leaf : {1, 2, 4, 8, 16, 32}) {
+      nb_config cfg{};
+      cfg.n = n;
+      cfg.eps2 = 40.f;
+      cfg.dt = 0.0035f;
+      cfg.solver = NB_SOLVER_BH;
+      cfg.leaf_size = leaf;
+      double ms = 0;
+      const int rc = time_forces(cfg, in, reps, &ms);
+      if (rc) return rc;
+      if (ms < best_ms) {
+        best_ms = ms;
+        best.leaf_size = leaf;
+        best.bh_ms = ms;
+      }
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_table[size_class(n)] = best;
+  }
+  if (out) *out = best;
+  return NB_OK;
+}
+
+extern "C" int nb_get_tuning(int64_t n, nb_tuning* out) {
+  if (!out || n < 1) return nb_fail(NB_E_ARG, "nb_get_tuning: bad argument");
+  nb_config cfg{};
+  cfg.n = n;
+  *out = nb_tuning{};
+  out->n = n;
+  int bpl = 0, js = 0, leaf = 0;
+  nb_tuned_shape(n, &bpl, &js, &leaf);
+  nb_default_shape(n, &bpl, &js, &leaf);
+  out->bodies_per_lane = bpl;
+  out->j_split = js;
+  out->leaf_size = leaf;
+  return NB_OK;
+}

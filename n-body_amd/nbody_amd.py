@@ -151,6 +151,21 @@ IC_PLUMMER = 0
 IC_BALL = 1
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED = 0, 1, 2, -1
 
+class Tuning(C.Structure):
+    _fields_ = [
+        ("n", C.c_int64),
+        ("bodies_per_lane", C.c_int32),
+        ("j_split", C.c_int32),
+        ("leaf_size", C.c_int32),
+        ("pad_", C.c_int32),
+        ("direct_ms", C.c_double),
+        ("bh_ms", C.c_double),
+    ]
+
+
+TUNE_DIRECT = 0x1
+TUNE_BH = 0x2
+
 _lib = None
 
 _P = C.c_void_p
@@ -196,6 +211,8 @@ _SIGS = {
     "nb_server_result": (C.c_int, [_P, C.c_int64, _P, _P]),
     "nb_server_forget": (C.c_int, [_P, C.c_int64]),
     "nb_server_now": (C.c_double, [_P]),
+    "nb_autotune": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.POINTER(Tuning)]),
+    "nb_get_tuning": (C.c_int, [C.c_int64, C.POINTER(Tuning)]),
     "nb_ic_ball": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, C.c_double, _P, _P, _P]),
     "nb_ic_plummer": (
         C.c_int,
@@ -245,6 +262,20 @@ def device_info() -> str:
     rc = load().nb_device_info(buf, 256)
     _check(rc)
     return buf.value.decode()
+
+
+def autotune(n: int, what: int = TUNE_DIRECT | TUNE_BH, reps: int = 3) -> dict:
+    """Time candidate launch shapes on the device and install the fastest as the
+    defaults for systems of n's size class (nb_autotune)."""
+    t = Tuning()
+    _check(load().nb_autotune(n, what, reps, C.byref(t)))
+    return {k: getattr(t, k) for k, _ in Tuning._fields_ if k != "pad_"}
+
+
+def get_tuning(n: int) -> dict:
+    t = Tuning()
+    _check(load().nb_get_tuning(n, C.byref(t)))
+    return {k: getattr(t, k) for k, _ in Tuning._fields_ if k != "pad_"}
 
 
 # ---- seeded initial conditions -------------------------------------------------

@@ -192,3 +192,23 @@ def test_timing_counters(nb):
         t = sim.timing()
     assert t["force_calls"] == 3 and t["step_calls"] == 3
     assert 0 < t["force_ms"] <= t["step_ms"]
+
+
+def test_autotune_installs_defaults(nb, ora):
+    before = nb.get_tuning(20000)
+    t = nb.autotune(20000, nb.TUNE_DIRECT | nb.TUNE_BH, reps=2)
+    assert t["bodies_per_lane"] in (1, 2, 4) and t["j_split"] in (1, 2, 4)
+    assert t["leaf_size"] in (1, 2, 4, 8, 16, 32)
+    assert t["direct_ms"] > 0 and t["bh_ms"] > 0
+    after = nb.get_tuning(30000)  # same power-of-two class
+    assert (after["bodies_per_lane"], after["j_split"], after["leaf_size"]) == \
+        (t["bodies_per_lane"], t["j_split"], t["leaf_size"])
+    assert before["leaf_size"] == 8  # built-in default before tuning
+    pos, vel, m = nb.ic_plummer(3, 20000, 20.0, 100.0)
+    with nb.Simulation(20000, 4.0, 0.001) as sim:  # uses the tuned shape
+        sim.upload(pos, vel, m)
+        f = sim.forces_f64()
+    tgt = np.arange(0, 20000, 97)
+    ref = ora.direct_f64(pos.astype(np.float32).astype(np.float64), m.astype(np.float64), 4.0,
+                         targets=tgt)
+    assert np.percentile(rel_err(f[tgt], ref), 99) < 1e-5

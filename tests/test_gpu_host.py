@@ -1,0 +1,69 @@
+"""GPU checks of the C++ drop-ins (ParticleSolverHIP / ParticleSolverBHutHIP via
+the headless CLI), the particle-file path and bit-exact snapshot resume."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parents[1]
+CLI = ROOT / "n-body_amd" / "bin" / "nbody_cli"
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def run_cli(*args):
+    r = subprocess.run([str(CLI), *map(str, args)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def test_cli_direct_matches_binding(nb, tmp_path):
+    out = tmp_path / "final.txt"
+    txt = run_cli("-v", 4, "-i", 5, "-n", 2048, "-t", 0.0035, "-s", 40, "--steps", 10, "--seed", 3,
+                  "--out", out, "--energy")
+    assert "interactions/s" in txt and "drift" in txt
+    p, v, a, m = nb.read_particle_file(out)
+    pos, vel, mass = nb.ic_ball(3, 2048, 100.0, 1.0)
+    with nb.Simulation(2048, 40.0, 0.0035) as sim:
+        sim.upload(nb.to_vec4(pos.astype(np.float32)), nb.to_vec4(vel.astype(np.float32)), mass)
+        sim.step(10)
+        gp, gv, ga = sim.download()
+    np.testing.assert_array_equal(p, gp)
+    np.testing.assert_array_equal(v, gv)
+    np.testing.assert_array_equal(a, ga)
+
+
+def test_cli_barnes_hut_from_reference_file(tmp_path):
+    out = tmp_path / "o.txt"
+    txt = run_cli("-v", 7, "-f", GOLDEN / "readme_particles.txt", "-t", 0.0035, "-s", 40,
+                  "--steps", 5, "--out", out)
+    assert "3 bodies" in txt
+    assert out.read_text().startswith("Particle System with 3 particles:")
+
+
+def test_snapshot_resume_is_bit_exact(nb, tmp_path):
+    for solver in (nb.SOLVER_DIRECT, nb.SOLVER_BH):
+        pos, vel, m = nb.ic_plummer(77, 5000, 20.0, 500.0)
+        snap = tmp_path / f"s{solver}.bin"
+        with nb.Simulation(5000, 4.0, 0.002, solver=solver) as sim:
+            sim.upload(pos, vel, m)
+            sim.step(6)
+            sim.save_snapshot(snap)
+            sim.step(6)
+            full = sim.download_f64()
+        with nb.Simulation(5000, 4.0, 0.002, solver=solver) as sim:
+            sim.load_snapshot(snap)
+            sim.step(6)
+            resumed = sim.download_f64()
+        for x, y in zip(full, resumed):
+            assert np.array_equal(x, y)
+
+
+def test_cli_snapshot_resume(tmp_path):
+    snap, a, b = tmp_path / "s.bin", tmp_path / "a.txt", tmp_path / "b.txt"
+    run_cli("-v", 4, "-i", 8, "-n", 3000, "--steps", 4, "--snapshot", snap)
+    run_cli("-v", 4, "-i", 8, "-n", 3000, "--steps", 4, "--resume", snap, "--out", a)
+    run_cli("-v", 4, "-i", 8, "-n", 3000, "--steps", 8, "--out", b)
+    assert a.read_text() == b.read_text()
