@@ -76,6 +76,10 @@ typedef struct nb_config {
 } nb_config;
 
 #define NB_F_NO_GRAPH 0x1 /* launch steps eagerly instead of replaying a captured hipGraph */
+/* Barnes-Hut: open cells per body, exactly the reference's test (Octree.cpp:320),
+ * instead of the default per-wave test on the bounding box of 64 Morton-consecutive
+ * bodies (never less accurate per body, no divergence). */
+#define NB_F_BH_PER_BODY 0x4
 
 typedef struct nb_ctx nb_ctx;
 

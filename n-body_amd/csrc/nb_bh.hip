@@ -633,6 +633,115 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ pms
   }
 }
 
+// Wave-coherent traversal (the default): the 64 lanes of a wave hold 64
+// consecutive Morton-sorted bodies and walk the tree together. A cell is
+// accepted for the whole wave when side^2 < theta^2 dmin^2, dmin = distance from
+// the cell's COM to the wave's bounding box; otherwise the wave descends, and a
+// leaf that is not accepted is evaluated body by body by every lane. The node
+// index is wave-uniform, so node and leaf-body data arrive through the scalar
+// cache and the branches are scalar: no divergence. Per body this accepts a
+// subset of what the per-body test accepts (at least as accurate).
+__device__ __forceinline__ float wave_uniform(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+template <bool kCount>
+__global__ __launch_bounds__(256) void k_traverse_group(
+    const float4* __restrict__ pms, const uint32_t* __restrict__ perm, int n,
+    const float4* __restrict__ com, const int4* __restrict__ link,
+    const int* __restrict__ level_off, int64_t max_nodes, const float* __restrict__ geom,
+    float theta2, float eps2, double G, double* __restrict__ force,
+    unsigned long long* __restrict__ inter, int* __restrict__ flags) {
+  const int t = blockIdx.x * 256 + threadIdx.x;
+  const bool valid = t < n;
+  const float4 me = pms[valid ? t : n - 1];
+  float lx = me.x, ly = me.y, lz = me.z, hx = me.x, hy = me.y, hz = me.z;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    lx = fminf(lx, __shfl_xor(lx, off));
+    ly = fminf(ly, __shfl_xor(ly, off));
+    lz = fminf(lz, __shfl_xor(lz, off));
+    hx = fmaxf(hx, __shfl_xor(hx, off));
+    hy = fmaxf(hy, __shfl_xor(hy, off));
+    hz = fmaxf(hz, __shfl_xor(hz, off));
+  }
+  lx = wave_uniform(lx);
+  ly = wave_uniform(ly);
+  lz = wave_uniform(lz);
+  hx = wave_uniform(hx);
+  hy = wave_uniform(hy);
+  hz = wave_uniform(hz);
+  const int total = (int)min((int64_t)level_off[kMaxLevel + 1], max_nodes);
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  unsigned long long cnt = 0;
+  int i = 0;
+  for (int it = 0; i >= 0; ++it) {
+    if (i >= total || it > total) {  // a rope walk visits each node once
+      if (threadIdx.x % 64 == 0) atomicOr(flags, 4);
+      break;
+    }
+    const float4 c = com[i];
+    const int4 nd = link[i];
+    const float ex = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.f);
+    const float ey = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.f);
+    const float ez = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.f);
+    const float dmin2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+    const bool accept = __builtin_amdgcn_readfirstlane(geom[6 + link_level(nd.w)] < theta2 * dmin2);
+    if (accept) {
+      if (c.w > 0.f) {
+        const float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
+        const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+        if (d2 > 0.f) {
+          const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
+          const float s = (c.w * inv) * (inv * inv);
+          ax = fmaf(dx, s, ax);
+          ay = fmaf(dy, s, ay);
+          az = fmaf(dz, s, az);
+          if (kCount) ++cnt;
+        }
+      }
+      i = nd.y;
+    } else if (nd.x < 0) {
+      const int e = min(nd.z + link_count(nd.w), n);
+      for (int b = max(nd.z, 0); b < e; ++b) {
+        const float4 q = pms[b];
+        const float bx = q.x - me.x, by = q.y - me.y, bz = q.z - me.z;
+        const float b2 = fmaf(bz, bz, fmaf(by, by, bx * bx));
+        if (b2 > 0.f) {
+          const float inv = __builtin_amdgcn_rsqf(b2 + eps2);
+          const float s = (q.w * inv) * (inv * inv);
+          ax = fmaf(bx, s, ax);
+          ay = fmaf(by, s, ay);
+          az = fmaf(bz, s, az);
+          if (kCount) ++cnt;
+        }
+      }
+      i = nd.y;
+    } else {
+      i = nd.x;
+    }
+    i = __builtin_amdgcn_readfirstlane(i);
+  }
+  if (valid) {
+    const uint32_t o = perm[t];
+    if (o < (uint32_t)n) {
+      force[o] = G * (double)ax;
+      force[n + o] = G * (double)ay;
+      force[2 * n + o] = G * (double)az;
+    }
+  }
+  if (kCount) {
+    __shared__ unsigned long long red[256];
+    red[threadIdx.x] = valid ? cnt : 0ull;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(inter, red[0]);
+  }
+}
+
 __global__ void k_zero_u64(unsigned long long* p) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = 0ull;
 }
@@ -765,18 +874,21 @@ hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions) {
   nb_bh_state& b = c->bh;
   const int n = c->n;
   const int g256 = (n + 255) / 256;
-  if (count_interactions) {
+  const bool group = !(c->cfg.flags & NB_F_BH_PER_BODY);
+  if (count_interactions)
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, b.inter);
-    hipLaunchKernelGGL((k_traverse<true>), dim3(g256), dim3(256), 0, c->stream, b.pms, b.vals[0],
-                       n, b.node_com, b.node_link, b.level_off, b.max_nodes, b.geom, c->theta2,
-                       c->cfg.eps2,
-                       c->G, c->force, b.inter, c->flags);
-  } else {
-    hipLaunchKernelGGL((k_traverse<false>), dim3(g256), dim3(256), 0, c->stream, b.pms, b.vals[0],
-                       n, b.node_com, b.node_link, b.level_off, b.max_nodes, b.geom, c->theta2,
-                       c->cfg.eps2,
-                       c->G, c->force, b.inter, c->flags);
-  }
+#define NB_TRAVERSE_ARGS                                                                       \
+  b.pms, b.vals[0], n, b.node_com, b.node_link, b.level_off, b.max_nodes, b.geom, c->theta2, \
+      c->cfg.eps2, c->G, c->force, b.inter, c->flags
+  if (group && count_interactions)
+    hipLaunchKernelGGL((k_traverse_group<true>), dim3(g256), dim3(256), 0, c->stream, NB_TRAVERSE_ARGS);
+  else if (group)
+    hipLaunchKernelGGL((k_traverse_group<false>), dim3(g256), dim3(256), 0, c->stream, NB_TRAVERSE_ARGS);
+  else if (count_interactions)
+    hipLaunchKernelGGL((k_traverse<true>), dim3(g256), dim3(256), 0, c->stream, NB_TRAVERSE_ARGS);
+  else
+    hipLaunchKernelGGL((k_traverse<false>), dim3(g256), dim3(256), 0, c->stream, NB_TRAVERSE_ARGS);
+#undef NB_TRAVERSE_ARGS
   return hipGetLastError();
 }
 

@@ -75,7 +75,8 @@ struct nb_server {
   std::vector<int64_t> running;   // admitted
   // initial-condition workers
   std::mutex mu;
-  std::condition_variable cv;
+  std::condition_variable cv;        // workers wait for jobs to prepare
+  std::condition_variable ready_cv;  // tick waits for the head job when idle
   std::deque<int64_t> prep_queue;
   int prepared_waiting = 0;
   bool stop = false;
@@ -107,7 +108,11 @@ struct nb_server {
               ? nb_ic_ball(j->desc.seed, n, 100.0, 1.0, j->pos.data(), j->vel.data(), j->mass.data())
               : nb_ic_plummer(j->desc.seed, n, 50.0, 1e4, 10.0, j->desc.G == 0.f ? 1.0 : j->desc.G,
                               j->pos.data(), j->vel.data(), j->mass.data());
-      j->prepared.store(rc == 0 ? 1 : -1);
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        j->prepared.store(rc == 0 ? 1 : -1);
+      }
+      ready_cv.notify_all();
     }
   }
 };
@@ -179,6 +184,7 @@ void free_server(nb_server* s) {
     s->stop = true;
   }
   s->cv.notify_all();
+  s->ready_cv.notify_all();
   for (auto& t : s->workers) t.join();
   for (auto& kv : s->jobs)
     if (kv.second->ctx) nb_destroy(kv.second->ctx);
@@ -261,6 +267,13 @@ int nb_server_submit(nb_server* s, const nb_job* job, int64_t* id) {
 int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
   if (!s) return nb_fail(NB_E_ARG, "nb_server_tick: null server");
   int finished = 0;
+  // Idle with queued work: wait (up to 50 ms) for the head job's initial
+  // conditions instead of returning an empty tick.
+  if (s->running.empty() && !s->queue.empty()) {
+    Job& head = *s->jobs.at(s->queue.front());
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->ready_cv.wait_for(lk, std::chrono::milliseconds(50), [&] { return head.prepared.load() != 0; });
+  }
   // 1. admit (arrival order, bounded look-ahead)
   int scanned = 0;
   for (auto it = s->queue.begin();

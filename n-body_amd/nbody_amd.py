@@ -33,6 +33,7 @@ SOLVER_BH = 1
 INTEG_KDK_LAGGED = 0
 INTEG_EULER = 1
 F_NO_GRAPH = 0x1
+F_BH_PER_BODY = 0x4
 
 
 class NBodyError(RuntimeError):

@@ -865,9 +865,94 @@ int64_t or_bh_forces(int64_t n, const float* pos4s, const uint32_t* perm, const 
   return total;
 }
 
-int64_t or_bh_pipeline(int64_t n, const float* pos4, float eps2, float G, float theta,
-                       int leaf_size, float* force4, uint32_t* codes_out, uint32_t* perm_out,
-                       int64_t* node_count, int nthreads) {
+int64_t or_bh_forces_group(int64_t n, const float* pos4s, const uint32_t* perm,
+                           const float bmin[3], const float bmax[3], int64_t nodes,
+                           const int32_t* first_child, const int32_t* next, const int32_t* start,
+                           const int32_t* nbody, const int32_t* level, const float* com4,
+                           float eps2, float G, float theta, int group, float* force4,
+                           int nthreads) {
+  (void)nodes;
+  if (group < 1 || group > 64) return -1;
+  float side2[11];
+  const float s0 = root_side(bmin, bmax);
+  for (int l = 0; l <= 10; ++l) {
+    const float s = ldexpf(s0, -l);
+    side2[l] = s * s;
+  }
+  const float theta2 = theta * theta;
+  const int64_t ngroups = (n + group - 1) / group;
+  int64_t total = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : total) num_threads(clamp_threads(nthreads))
+  for (int64_t g = 0; g < ngroups; ++g) {
+    const int64_t t0 = g * group, t1 = (t0 + group < n) ? t0 + group : n;
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (int64_t t = t0; t < t1; ++t)
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = fminf(lo[a], pos4s[4 * t + a]);
+        hi[a] = fmaxf(hi[a], pos4s[4 * t + a]);
+      }
+    float acc[64][3];
+    int64_t cnt = 0;
+    memset(acc, 0, sizeof(acc));
+    int32_t i = 0;
+    while (i >= 0) {
+      const float* c = &com4[4 * i];
+      /* squared distance from the COM to the group's box */
+      float e[3];
+      for (int a = 0; a < 3; ++a) e[a] = fmaxf(fmaxf(lo[a] - c[a], c[a] - hi[a]), 0.f);
+      const float dmin2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+      if (side2[level[i]] < theta2 * dmin2) {
+        if (c[3] > 0.f) {
+          for (int64_t t = t0; t < t1; ++t) {
+            const float dx = c[0] - pos4s[4 * t], dy = c[1] - pos4s[4 * t + 1], dz = c[2] - pos4s[4 * t + 2];
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 > 0.f) {
+              const float inv = 1.0f / sqrtf(d2 + eps2);
+              const float s = c[3] * inv * inv * inv;
+              acc[t - t0][0] += dx * s;
+              acc[t - t0][1] += dy * s;
+              acc[t - t0][2] += dz * s;
+              ++cnt;
+            }
+          }
+        }
+        i = next[i];
+      } else if (first_child[i] < 0) {
+        const int64_t e1 = (int64_t)start[i] + nbody[i];
+        for (int64_t b = start[i]; b < e1; ++b)
+          for (int64_t t = t0; t < t1; ++t) {
+            const float bx = pos4s[4 * b] - pos4s[4 * t], by = pos4s[4 * b + 1] - pos4s[4 * t + 1];
+            const float bz = pos4s[4 * b + 2] - pos4s[4 * t + 2];
+            const float b2 = bx * bx + by * by + bz * bz;
+            if (b2 > 0.f) {
+              const float inv = 1.0f / sqrtf(b2 + eps2);
+              const float s = pos4s[4 * b + 3] * inv * inv * inv;
+              acc[t - t0][0] += bx * s;
+              acc[t - t0][1] += by * s;
+              acc[t - t0][2] += bz * s;
+              ++cnt;
+            }
+          }
+        i = next[i];
+      } else {
+        i = first_child[i];
+      }
+    }
+    for (int64_t t = t0; t < t1; ++t) {
+      const int64_t o = perm[t];
+      force4[4 * o] = G * acc[t - t0][0];
+      force4[4 * o + 1] = G * acc[t - t0][1];
+      force4[4 * o + 2] = G * acc[t - t0][2];
+      force4[4 * o + 3] = 0.f;
+    }
+    total += cnt;
+  }
+  return total;
+}
+
+int64_t or_bh_pipeline_group(int64_t n, const float* pos4, float eps2, float G, float theta,
+                             int leaf_size, int group, float* force4, uint32_t* codes_out,
+                             uint32_t* perm_out, int64_t* node_count, int nthreads) {
   float bmin[3], bmax[3];
   or_bbox(n, pos4, bmin, bmax);
   uint32_t* codes = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
@@ -887,12 +972,22 @@ int64_t or_bh_pipeline(int64_t n, const float* pos4, float eps2, float G, float 
   float* com = (float*)malloc(sizeof(float) * 4 * (size_t)bound);
   const int64_t nodes = or_build_tree(n, sorted, leaf_size, bound, fc, nx, st, nb, lv);
   or_tree_com(nodes, st, nb, n, pos4s, com);
-  const int64_t inter = or_bh_forces(n, pos4s, perm, bmin, bmax, nodes, fc, nx, st, nb, lv, com,
-                                     eps2, G, theta, force4, nthreads);
+  const int64_t inter =
+      group > 0 ? or_bh_forces_group(n, pos4s, perm, bmin, bmax, nodes, fc, nx, st, nb, lv, com, eps2,
+                                     G, theta, group, force4, nthreads)
+                : or_bh_forces(n, pos4s, perm, bmin, bmax, nodes, fc, nx, st, nb, lv, com, eps2, G,
+                               theta, force4, nthreads);
   if (codes_out) memcpy(codes_out, codes, sizeof(uint32_t) * (size_t)n);
   if (perm_out) memcpy(perm_out, perm, sizeof(uint32_t) * (size_t)n);
   if (node_count) *node_count = nodes;
   free(codes); free(sorted); free(perm); free(pos4s);
   free(fc); free(nx); free(st); free(nb); free(lv); free(com);
   return inter;
+}
+
+int64_t or_bh_pipeline(int64_t n, const float* pos4, float eps2, float G, float theta,
+                       int leaf_size, float* force4, uint32_t* codes_out, uint32_t* perm_out,
+                       int64_t* node_count, int nthreads) {
+  return or_bh_pipeline_group(n, pos4, eps2, G, theta, leaf_size, 0, force4, codes_out, perm_out,
+                              node_count, nthreads);
 }

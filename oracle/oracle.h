@@ -125,11 +125,27 @@ int64_t or_bh_forces(int64_t n, const float* pos4s, const uint32_t* perm, const 
                      const int32_t* next, const int32_t* start, const int32_t* nbody,
                      const int32_t* level, const float* com4, float eps2, float G, float theta,
                      float* force4, int nthreads);
+/* Group traversal (the engine's default on the GPU): bodies are taken in
+ * groups of `group` consecutive sorted bodies; a cell is accepted for the whole
+ * group when s_l^2 < theta^2 dmin^2, dmin = distance from the cell's COM to the
+ * group's bounding box, else the group descends; a leaf that is not accepted is
+ * evaluated body by body for every member. Per body this accepts a subset of the
+ * cells the per-body test accepts, so it is at least as accurate. */
+int64_t or_bh_forces_group(int64_t n, const float* pos4s, const uint32_t* perm,
+                           const float bmin[3], const float bmax[3], int64_t nodes,
+                           const int32_t* first_child, const int32_t* next, const int32_t* start,
+                           const int32_t* nbody, const int32_t* level, const float* com4,
+                           float eps2, float G, float theta, int group, float* force4,
+                           int nthreads);
 /* The whole engine BH force pass on the CPU: bbox, Morton, sort, tree, COM,
  * traversal. Node arrays are optional outputs (NULL to skip). */
 int64_t or_bh_pipeline(int64_t n, const float* pos4, float eps2, float G, float theta,
                        int leaf_size, float* force4, uint32_t* codes, uint32_t* perm,
                        int64_t* node_count, int nthreads);
+/* As or_bh_pipeline with the group traversal (group = 0: per-body traversal). */
+int64_t or_bh_pipeline_group(int64_t n, const float* pos4, float eps2, float G, float theta,
+                             int leaf_size, int group, float* force4, uint32_t* codes,
+                             uint32_t* perm, int64_t* node_count, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -37,6 +37,10 @@ _SIGS = {
     "or_tree_com": (None, [_I, _P, _P, _I, _P, _P]),
     "or_bh_forces": (_I, [_I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, C.c_float, C.c_float,
                           C.c_float, _P, C.c_int]),
+    "or_bh_forces_group": (_I, [_I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, C.c_float, C.c_float,
+                                C.c_float, C.c_int, _P, C.c_int]),
+    "or_bh_pipeline_group": (_I, [_I, _P, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, _P, _P, _P,
+                                  _P, C.c_int]),
     "or_bh_pipeline": (_I, [_I, _P, C.c_float, C.c_float, C.c_float, C.c_int, _P, _P, _P, _P, C.c_int]),
 }
 _libs: dict[bool, C.CDLL] = {}
@@ -205,13 +209,28 @@ def bh_forces(pos4s, perm, bmin, bmax, tree, com4, eps2, theta=0.5, G=1.0, nthre
     return out, int(inter)
 
 
-def bh_pipeline(pos4, eps2, theta=0.5, leaf_size=8, G=1.0, nthreads=None):
+def bh_pipeline(pos4, eps2, theta=0.5, leaf_size=8, G=1.0, nthreads=None, group=0):
+    """The engine's Barnes-Hut force pass on the CPU. group=64 is the GPU's default
+    wave traversal; group=0 the per-body traversal."""
     pos4 = np.ascontiguousarray(pos4, np.float32)
     n = pos4.shape[0]
     out = np.zeros((n, 4), np.float32)
     codes = np.empty(n, np.uint32)
     perm = np.empty(n, np.uint32)
     cnt = C.c_int64()
-    inter = load().or_bh_pipeline(n, _p(pos4), eps2, G, theta, leaf_size, _p(out), _p(codes), _p(perm),
-                                  C.byref(cnt), nthreads or threads())
+    inter = load().or_bh_pipeline_group(n, _p(pos4), eps2, G, theta, leaf_size, group, _p(out),
+                                        _p(codes), _p(perm), C.byref(cnt), nthreads or threads())
     return out, codes, perm, cnt.value, int(inter)
+
+
+def bh_forces_group(pos4s, perm, bmin, bmax, tree, com4, eps2, theta=0.5, G=1.0, group=64,
+                    nthreads=None):
+    n = pos4s.shape[0]
+    out = np.zeros((n, 4), np.float32)
+    inter = load().or_bh_forces_group(
+        n, _p(np.ascontiguousarray(pos4s, np.float32)), _p(np.ascontiguousarray(perm, np.uint32)),
+        _p(np.asarray(bmin, np.float32)), _p(np.asarray(bmax, np.float32)), tree["start"].shape[0],
+        _p(tree["first_child"]), _p(tree["next"]), _p(tree["start"]), _p(tree["count"]),
+        _p(tree["level"]), _p(np.ascontiguousarray(com4, np.float32)), eps2, G, theta, group, _p(out),
+        nthreads or threads())
+    return out, int(inter)

@@ -25,8 +25,9 @@ def cpu_build(ora, pos4, leaf):
     return mn, mx, codes, perm, tree, com
 
 
-def gpu_run(nb, pos, vel, m, leaf, theta=0.5):
-    with nb.Simulation(len(m), EPS2, DT, solver=nb.SOLVER_BH, leaf_size=leaf, theta=theta) as sim:
+def gpu_run(nb, pos, vel, m, leaf, theta=0.5, flags=0):
+    with nb.Simulation(len(m), EPS2, DT, solver=nb.SOLVER_BH, leaf_size=leaf, theta=theta,
+                       flags=flags) as sim:
         sim.upload(pos, vel, m)
         f = sim.forces()
         codes, perm = sim.debug_morton()
@@ -62,24 +63,36 @@ def test_bit_exact_topology_and_forces(nb, ora, leaf):
     np.testing.assert_array_equal(nodes["level"], tree["level"])
     np.testing.assert_allclose(nodes["mass"], com[:, 3], rtol=1e-6)
     np.testing.assert_allclose(nodes["com"], com[:, :3], rtol=1e-6, atol=1e-5)
-    fc, cinter = ora.bh_forces(p4[cperm], cperm, mn, mx, tree, com, EPS2)
+    # default wave traversal vs its CPU twin (groups of 64 sorted bodies)
+    fc, cinter = ora.bh_forces_group(p4[cperm], cperm, mn, mx, tree, com, EPS2, group=64)
     e = rel_err(f, fc)
     print(f"leaf {leaf}: GPU vs CPU build p50={np.median(e):.2e} p99={np.percentile(e, 99):.2e}; "
           f"interactions {inter} vs {cinter}")
     assert np.percentile(e, 99) < 1e-5
     assert abs(inter - cinter) <= 1e-4 * cinter
+    # the reference's per-body test (NB_F_BH_PER_BODY) vs the per-body CPU traversal
+    fp, *_, pinter = gpu_run(nb, pos, vel, m, leaf, flags=nb.F_BH_PER_BODY)
+    fcp, cpinter = ora.bh_forces(p4[cperm], cperm, mn, mx, tree, com, EPS2)
+    assert np.percentile(rel_err(fp, fcp), 99) < 1e-5
+    assert abs(pinter - cpinter) <= 1e-4 * cpinter
+    assert inter >= pinter
 
 
 def test_accuracy_vs_fp64(nb, ora):
     n = 50000
     pos, vel, m = nb.ic_plummer(11, n, 50.0, 1e4)
-    f, *_ = gpu_run(nb, pos, vel, m, 8)
     tgt = np.random.default_rng(1).choice(n, 2000, replace=False)
     ref = ora.direct_f64(pos.astype(np.float32).astype(np.float64), m.astype(np.float64), EPS2,
                          targets=tgt)
-    e = rel_err(f[tgt], ref)
-    print(f"BH theta=0.5 N=50000 vs FP64: p50={np.median(e):.2e} p99={np.percentile(e, 99):.2e}")
-    assert np.median(e) < 5e-3 and np.percentile(e, 99) < 3e-2
+    errs = {}
+    for name, flags in (("wave", 0), ("per-body", nb.F_BH_PER_BODY)):
+        f, *_ = gpu_run(nb, pos, vel, m, 8, flags=flags)
+        e = rel_err(f[tgt], ref)
+        errs[name] = np.median(e)
+        print(f"BH theta=0.5 N=50000 {name} vs FP64: p50={np.median(e):.2e} "
+              f"p99={np.percentile(e, 99):.2e}")
+        assert np.median(e) < 5e-3 and np.percentile(e, 99) < 3e-2
+    assert errs["wave"] <= errs["per-body"]
 
 
 def test_theta_tiny_is_direct_sum(nb, ora):
@@ -123,7 +136,8 @@ def test_steps_kdk_vs_cpu_pipeline(nb, ora):
         p, v, a = sim.download_f64()
     op, ov, oa = pos.copy(), vel.copy(), np.zeros_like(pos)
     for _ in range(5):
-        f, *_ = ora.bh_pipeline(pm4(op.astype(np.float32), m), EPS2, theta=0.5, leaf_size=8)
+        f, *_ = ora.bh_pipeline(pm4(op.astype(np.float32), m), EPS2, theta=0.5, leaf_size=8,
+                                group=64)
         ora.kdk_update_f64(op, ov, oa, f[:, :3].astype(np.float64), DT)
     assert np.abs(p - op).max() < 1e-6 * np.abs(op).max()
     assert rel_err(a, oa).max() < 1e-3

@@ -254,3 +254,20 @@ def test_v6_small_inputs(ora):
         assert rc == 0 and np.all(np.isfinite(f))
         if n <= 9:
             assert rel_err(f, ref).max() < 2e-2
+
+
+def test_group_traversal_limits_and_accuracy(ora):
+    pos = vec4(seeded_ball(3000, seed=22))
+    pos[:, 3] = 1.0
+    ref = ora.direct_f64(pos[:, :3].astype(np.float64), np.ones(3000), 40.0)
+    # theta -> 0: every leaf, every pair
+    f0, *_, inter0 = ora.bh_pipeline(pos, 40.0, theta=1e-7, leaf_size=8, group=64)
+    assert inter0 == 3000 * 2999 and rel_err(f0, ref).max() < 1e-5
+    # the group test is stricter than the per-body test: at least as accurate, more pairs
+    fb, *_, ib = ora.bh_pipeline(pos, 40.0, theta=0.5, leaf_size=8, group=0)
+    fg, *_, ig = ora.bh_pipeline(pos, 40.0, theta=0.5, leaf_size=8, group=64)
+    assert ig >= ib
+    assert np.median(rel_err(fg, ref)) <= np.median(rel_err(fb, ref))
+    # group of one body is the per-body traversal
+    f1, *_, i1 = ora.bh_pipeline(pos, 40.0, theta=0.5, leaf_size=8, group=1)
+    assert i1 == ib and rel_err(f1, fb).max() < 1e-6
