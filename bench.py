@@ -104,7 +104,7 @@ def bench_bh(nb, steps: int, warmup: int):
     t0 = time.perf_counter()
     pos, vel, m = nb.ic_discs(2024, n)
     log(f"bh: discs generated ({time.perf_counter() - t0:.1f} s)")
-    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH, leaf_size=8) as sim:
+    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH) as sim:  # tuned leaf size
         sim.upload(pos, vel, m)
         inter = sim.count_interactions()
         sim.step(warmup)
@@ -118,7 +118,8 @@ def bench_bh(nb, steps: int, warmup: int):
     ms = 1e3 * wall / steps
     return {
         "workload": "Barnes-Hut N=4194304, two exponential discs (Rd=30, h=2, 300 apart, "
-                    "closing 0.5/step), theta=0.5, leaf=8, eps2=40, dt=0.0035",
+                    "closing 0.5/step), theta=0.5, eps2=40, dt=0.0035",
+        "leaf_size": nb.get_tuning(n)["leaf_size"],
         "steps": steps,
         "ms_per_step": ms,
         "build_ms": t["build_ms"] / max(1, t["build_calls"]),

@@ -80,6 +80,10 @@ typedef struct nb_config {
  * instead of the default per-wave test on the bounding box of 64 Morton-consecutive
  * bodies (never less accurate per body, no divergence). */
 #define NB_F_BH_PER_BODY 0x4
+/* Barnes-Hut: quantize each axis over its own extent, exactly the inputs of the
+ * reference's morton.glsl:68-75 (cells become boxes with the root's aspect
+ * ratio), instead of the default cubic root cell (extent = the largest axis). */
+#define NB_F_BH_AXIS_BOX 0x8
 
 typedef struct nb_ctx nb_ctx;
 

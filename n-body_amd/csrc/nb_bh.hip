@@ -48,6 +48,7 @@ constexpr int kSortItems = 16;              // elements per lane per pass
 constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 elements per block
 constexpr int kScanTile = 2048;             // elements per block in the scans
 constexpr int kScanThreads = 256;
+constexpr int kSpare = 4;  // zero-mass bodies after the sorted body array
 
 // ---- 1. bounding box --------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bbox_partial(const float4* __restrict__ pm, int n,
@@ -112,6 +113,7 @@ __global__ __launch_bounds__(256) void k_bbox_final(const float* __restrict__ pa
       const float s = ldexpf(side, -l);
       geom[6 + l] = __fmul_rn(s, s);
     }
+    geom[6 + kMaxLevel + 1] = side;  // cubic root extent
   }
 }
 
@@ -124,15 +126,19 @@ __device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
   return v;
 }
 
-__device__ __forceinline__ uint32_t quantize(float p, float lo, float hi) {
-  const float ext = fmaxf(__fsub_rn(hi, lo), 1e-6f);
+// morton.glsl:68-75 and :49-51 with the extent given: n = clamp((p - lo) / ext, 0, 1),
+// q = uint(clamp(n * 1024, 0, 1023)).
+__device__ __forceinline__ uint32_t quantize(float p, float lo, float ext) {
   float nrm = __fdiv_rn(__fsub_rn(p, lo), ext);
   nrm = fminf(fmaxf(nrm, 0.f), 1.f);
   const float g = fminf(fmaxf(__fmul_rn(nrm, 1024.f), 0.f), 1023.f);
   return (uint32_t)g;
 }
 
-__global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ pm, int n,
+// cubic = 1: one extent E = max_a max(max_a - min_a, 1e-6) on every axis (a cubic
+// root cell, so every cell is a cube of side E / 2^level); cubic = 0: each axis
+// over its own extent max(max_a - min_a, 1e-6), exactly morton.glsl's inputs.
+__global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ pm, int n, int cubic,
                                                 const float* __restrict__ geom,
                                                 uint32_t* __restrict__ codes,
                                                 uint32_t* __restrict__ keys,
@@ -140,9 +146,13 @@ __global__ __launch_bounds__(256) void k_morton(const float4* __restrict__ pm, i
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const float4 p = pm[i];
-  const uint32_t qx = quantize(p.x, geom[0], geom[3]);
-  const uint32_t qy = quantize(p.y, geom[1], geom[4]);
-  const uint32_t qz = quantize(p.z, geom[2], geom[5]);
+  const float E = geom[6 + kMaxLevel + 1];
+  const float ex = cubic ? E : fmaxf(__fsub_rn(geom[3], geom[0]), 1e-6f);
+  const float ey = cubic ? E : fmaxf(__fsub_rn(geom[4], geom[1]), 1e-6f);
+  const float ez = cubic ? E : fmaxf(__fsub_rn(geom[5], geom[2]), 1e-6f);
+  const uint32_t qx = quantize(p.x, geom[0], ex);
+  const uint32_t qy = quantize(p.y, geom[1], ey);
+  const uint32_t qz = quantize(p.z, geom[2], ez);
   const uint32_t code = expand_bits(qx) * 4u + expand_bits(qy) * 2u + expand_bits(qz);
   codes[i] = code;
   keys[i] = code;
@@ -702,18 +712,25 @@ __global__ __launch_bounds__(256) void k_traverse_group(
       }
       i = nd.y;
     } else if (nd.x < 0) {
+      // leaf bodies four at a time (one scalar load); bodies past the leaf's end
+      // (the next leaf, or the zero spare entries) get mass 0
       const int e = min(nd.z + link_count(nd.w), n);
-      for (int b = max(nd.z, 0); b < e; ++b) {
-        const float4 q = pms[b];
-        const float bx = q.x - me.x, by = q.y - me.y, bz = q.z - me.z;
-        const float b2 = fmaf(bz, bz, fmaf(by, by, bx * bx));
-        if (b2 > 0.f) {
+      for (int b = max(nd.z, 0); b < e; b += 4) {
+        float4 q[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) q[u] = pms[b + u];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          const float w = b + u < e ? q[u].w : 0.f;
+          const float bx = q[u].x - me.x, by = q[u].y - me.y, bz = q[u].z - me.z;
+          const float b2 = fmaf(bz, bz, fmaf(by, by, bx * bx));
           const float inv = __builtin_amdgcn_rsqf(b2 + eps2);
-          const float s = (q.w * inv) * (inv * inv);
+          float s = (w * inv) * (inv * inv);
+          s = b2 > 0.f ? s : 0.f;
           ax = fmaf(bx, s, ax);
           ay = fmaf(by, s, ay);
           az = fmaf(bz, s, az);
-          if (kCount) ++cnt;
+          if (kCount) cnt += (b + u < e && b2 > 0.f) ? 1 : 0;
         }
       }
       i = nd.y;
@@ -788,7 +805,7 @@ int nb_bh_alloc(nb_ctx* c) {
       A((void**)&b.vals[0], sizeof(uint32_t) * n) || A((void**)&b.vals[1], sizeof(uint32_t) * n) ||
       A((void**)&b.codes, sizeof(uint32_t) * n) ||
       A((void**)&b.digit_hist, sizeof(int) * 2 * hist) ||
-      A((void**)&b.pms, sizeof(float4) * n) ||
+      A((void**)&b.pms, sizeof(float4) * (n + kSpare)) ||
       A((void**)&b.prefix, sizeof(double4) * (n + 1)) ||
       A((void**)&b.scan_part, sizeof(double4) * ((n + kScanTile - 1) / kScanTile + 2)) ||
       A((void**)&b.node_com, sizeof(float4) * bound) ||
@@ -799,6 +816,10 @@ int nb_bh_alloc(nb_ctx* c) {
       A((void**)&b.counters, sizeof(int) * 16) ||
       A((void**)&b.inter, sizeof(unsigned long long) * 2))
     return nb_fail(NB_E_OOM, "nb_create: device allocation failed (octree)");
+  // the batched leaf loop may read up to kSpare bodies past the end: zero mass
+  if (hipMemset(b.pms + n, 0, sizeof(float4) * kSpare) != hipSuccess ||
+      hipMemset(b.geom, 0, sizeof(float) * 32) != hipSuccess)
+    return nb_fail(NB_E_HIP, "nb_create: hipMemset failed (octree)");
   return NB_OK;
 }
 
@@ -822,7 +843,8 @@ hipError_t nb_bh_build(nb_ctx* c) {
   hipLaunchKernelGGL(k_bbox_partial, dim3(b.bbox_blocks), dim3(256), 0, s, c->pm, n, b.bbox_part);
   hipLaunchKernelGGL(k_bbox_final, dim3(1), dim3(256), 0, s, b.bbox_part, b.bbox_blocks, b.geom);
   // 2. encode
-  hipLaunchKernelGGL(k_morton, dim3(g256), dim3(256), 0, s, c->pm, n, b.geom, b.codes, b.keys[0],
+  const int cubic = (c->cfg.flags & NB_F_BH_AXIS_BOX) ? 0 : 1;
+  hipLaunchKernelGGL(k_morton, dim3(g256), dim3(256), 0, s, c->pm, n, cubic, b.geom, b.codes, b.keys[0],
                      b.vals[0]);
   // 3. sort: 4 passes of 8 bits, ping-pong 0 -> 1 -> 0 -> 1 -> 0
   const int nb = b.sort_blocks;

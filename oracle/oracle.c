@@ -672,21 +672,30 @@ uint32_t or_expand_bits(uint32_t v) {
 }
 
 /* Quantize one coordinate: morton.glsl:49-51 and :68-75. */
-static uint32_t quantize(float p, float lo, float hi) {
-  float ext = hi - lo;
-  ext = ext > 1e-6f ? ext : 1e-6f;               /* max(extent, 1e-6) */
+static float axis_extent(float lo, float hi) {
+  const float ext = hi - lo;
+  return ext > 1e-6f ? ext : 1e-6f;              /* max(extent, 1e-6) */
+}
+
+static uint32_t quantize(float p, float lo, float ext) {
   float nrm = (p - lo) / ext;                    /* IEEE division */
   nrm = fminf(fmaxf(nrm, 0.f), 1.f);             /* clamp(n, 0, 1) */
   const float g = fminf(fmaxf(nrm * 1024.f, 0.f), 1023.f);
   return (uint32_t)g;                            /* uint() truncates */
 }
 
-void or_morton(int64_t n, const float* pos4, const float bmin[3], const float bmax[3],
+void or_morton(int64_t n, const float* pos4, const float bmin[3], const float bmax[3], int cubic,
                uint32_t* codes) {
+  float ext[3];
+  for (int a = 0; a < 3; ++a) ext[a] = axis_extent(bmin[a], bmax[a]);
+  if (cubic) {
+    const float e = fmaxf(fmaxf(ext[0], ext[1]), ext[2]);
+    ext[0] = ext[1] = ext[2] = e;
+  }
   for (int64_t i = 0; i < n; ++i) {
-    const uint32_t qx = quantize(pos4[4 * i], bmin[0], bmax[0]);
-    const uint32_t qy = quantize(pos4[4 * i + 1], bmin[1], bmax[1]);
-    const uint32_t qz = quantize(pos4[4 * i + 2], bmin[2], bmax[2]);
+    const uint32_t qx = quantize(pos4[4 * i], bmin[0], ext[0]);
+    const uint32_t qy = quantize(pos4[4 * i + 1], bmin[1], ext[1]);
+    const uint32_t qz = quantize(pos4[4 * i + 2], bmin[2], ext[2]);
     codes[i] = or_expand_bits(qx) * 4u + or_expand_bits(qy) * 2u + or_expand_bits(qz);
   }
 }
@@ -951,14 +960,14 @@ int64_t or_bh_forces_group(int64_t n, const float* pos4s, const uint32_t* perm,
 }
 
 int64_t or_bh_pipeline_group(int64_t n, const float* pos4, float eps2, float G, float theta,
-                             int leaf_size, int group, float* force4, uint32_t* codes_out,
+                             int leaf_size, int group, int cubic, float* force4, uint32_t* codes_out,
                              uint32_t* perm_out, int64_t* node_count, int nthreads) {
   float bmin[3], bmax[3];
   or_bbox(n, pos4, bmin, bmax);
   uint32_t* codes = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
   uint32_t* sorted = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
   uint32_t* perm = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)n);
-  or_morton(n, pos4, bmin, bmax, codes);
+  or_morton(n, pos4, bmin, bmax, cubic, codes);
   or_sort_codes(n, codes, perm, sorted);
   float* pos4s = (float*)malloc(sizeof(float) * 4 * (size_t)n);
   for (int64_t k = 0; k < n; ++k)
@@ -988,6 +997,6 @@ int64_t or_bh_pipeline_group(int64_t n, const float* pos4, float eps2, float G, 
 int64_t or_bh_pipeline(int64_t n, const float* pos4, float eps2, float G, float theta,
                        int leaf_size, float* force4, uint32_t* codes_out, uint32_t* perm_out,
                        int64_t* node_count, int nthreads) {
-  return or_bh_pipeline_group(n, pos4, eps2, G, theta, leaf_size, 0, force4, codes_out, perm_out,
-                              node_count, nthreads);
+  return or_bh_pipeline_group(n, pos4, eps2, G, theta, leaf_size, 0, 0, force4, codes_out,
+                              perm_out, node_count, nthreads);
 }

@@ -87,10 +87,12 @@ int or_v6_forces(int64_t n, const float* pos4, const float* mass4, float eps2, f
 /* ---- CPU build of the engine's Morton / sort / octree (topology reference) ---- */
 /* Exact min/max of the FP32 positions; max starts at -FLT_MAX. */
 void or_bbox(int64_t n, const float* pos4, float bmin[3], float bmax[3]);
-/* 30-bit Morton code per morton.glsl:37-75: n = clamp((p-min)/max(max-min,1e-6),0,1);
- * q = uint(clamp(n*1024, 0, 1023)); code = expand(qx)*4 + expand(qy)*2 + expand(qz). */
+/* 30-bit Morton code per morton.glsl:37-75: n = clamp((p-min)/ext,0,1);
+ * q = uint(clamp(n*1024, 0, 1023)); code = expand(qx)*4 + expand(qy)*2 + expand(qz).
+ * cubic = 0: ext = max(max-min, 1e-6) per axis (morton.glsl's inputs);
+ * cubic = 1: one ext = the largest of those on every axis (the engine's default). */
 uint32_t or_expand_bits(uint32_t v);
-void or_morton(int64_t n, const float* pos4, const float bmin[3], const float bmax[3],
+void or_morton(int64_t n, const float* pos4, const float bmin[3], const float bmax[3], int cubic,
                uint32_t* codes);
 /* Stable sort of (code, index): perm[k] = index of the k-th smallest (code, index). */
 void or_sort_codes(int64_t n, const uint32_t* codes, uint32_t* perm, uint32_t* codes_sorted);
@@ -142,9 +144,11 @@ int64_t or_bh_forces_group(int64_t n, const float* pos4s, const uint32_t* perm,
 int64_t or_bh_pipeline(int64_t n, const float* pos4, float eps2, float G, float theta,
                        int leaf_size, float* force4, uint32_t* codes, uint32_t* perm,
                        int64_t* node_count, int nthreads);
-/* As or_bh_pipeline with the group traversal (group = 0: per-body traversal). */
+/* As or_bh_pipeline with the group traversal (group = 0: per-body traversal) and
+ * the choice of Morton box (cubic, see or_morton); or_bh_pipeline is group 0,
+ * cubic 0. */
 int64_t or_bh_pipeline_group(int64_t n, const float* pos4, float eps2, float G, float theta,
-                             int leaf_size, int group, float* force4, uint32_t* codes,
+                             int leaf_size, int group, int cubic, float* force4, uint32_t* codes,
                              uint32_t* perm, int64_t* node_count, int nthreads);
 
 #ifdef __cplusplus

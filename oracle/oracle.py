@@ -30,7 +30,7 @@ _SIGS = {
                                C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "or_bbox": (None, [_I, _P, _P, _P]),
     "or_expand_bits": (C.c_uint32, [C.c_uint32]),
-    "or_morton": (None, [_I, _P, _P, _P, _P]),
+    "or_morton": (None, [_I, _P, _P, _P, C.c_int, _P]),
     "or_sort_codes": (None, [_I, _P, _P, _P]),
     "or_tree_bound": (_I, [_I]),
     "or_build_tree": (_I, [_I, _P, C.c_int, _I, _P, _P, _P, _P, _P]),
@@ -39,8 +39,8 @@ _SIGS = {
                           C.c_float, _P, C.c_int]),
     "or_bh_forces_group": (_I, [_I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, C.c_float, C.c_float,
                                 C.c_float, C.c_int, _P, C.c_int]),
-    "or_bh_pipeline_group": (_I, [_I, _P, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, _P, _P, _P,
-                                  _P, C.c_int]),
+    "or_bh_pipeline_group": (_I, [_I, _P, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
+                                  _P, _P, _P, _P, C.c_int]),
     "or_bh_pipeline": (_I, [_I, _P, C.c_float, C.c_float, C.c_float, C.c_int, _P, _P, _P, _P, C.c_int]),
 }
 _libs: dict[bool, C.CDLL] = {}
@@ -163,11 +163,11 @@ def expand_bits(v: int) -> int:
     return int(load().or_expand_bits(v))
 
 
-def morton(pos4, bmin, bmax):
+def morton(pos4, bmin, bmax, cubic=False):
     pos4 = np.ascontiguousarray(pos4, np.float32)
     codes = np.empty(pos4.shape[0], np.uint32)
     load().or_morton(pos4.shape[0], _p(pos4), _p(np.asarray(bmin, np.float32)),
-                     _p(np.asarray(bmax, np.float32)), _p(codes))
+                     _p(np.asarray(bmax, np.float32)), 1 if cubic else 0, _p(codes))
     return codes
 
 
@@ -209,17 +209,18 @@ def bh_forces(pos4s, perm, bmin, bmax, tree, com4, eps2, theta=0.5, G=1.0, nthre
     return out, int(inter)
 
 
-def bh_pipeline(pos4, eps2, theta=0.5, leaf_size=8, G=1.0, nthreads=None, group=0):
-    """The engine's Barnes-Hut force pass on the CPU. group=64 is the GPU's default
-    wave traversal; group=0 the per-body traversal."""
+def bh_pipeline(pos4, eps2, theta=0.5, leaf_size=8, G=1.0, nthreads=None, group=0, cubic=False):
+    """The engine's Barnes-Hut force pass on the CPU. The GPU's defaults are
+    group=64 (wave traversal) and cubic=True; group=0 is the per-body traversal."""
     pos4 = np.ascontiguousarray(pos4, np.float32)
     n = pos4.shape[0]
     out = np.zeros((n, 4), np.float32)
     codes = np.empty(n, np.uint32)
     perm = np.empty(n, np.uint32)
     cnt = C.c_int64()
-    inter = load().or_bh_pipeline_group(n, _p(pos4), eps2, G, theta, leaf_size, group, _p(out),
-                                        _p(codes), _p(perm), C.byref(cnt), nthreads or threads())
+    inter = load().or_bh_pipeline_group(n, _p(pos4), eps2, G, theta, leaf_size, group,
+                                        1 if cubic else 0, _p(out), _p(codes), _p(perm), C.byref(cnt),
+                                        nthreads or threads())
     return out, codes, perm, cnt.value, int(inter)
 
 

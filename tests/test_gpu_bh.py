@@ -16,9 +16,9 @@ pytestmark = pytest.mark.gpu
 EPS2, DT = 40.0, 0.0035
 
 
-def cpu_build(ora, pos4, leaf):
+def cpu_build(ora, pos4, leaf, cubic=True):
     mn, mx = ora.bbox(pos4)
-    codes = ora.morton(pos4, mn, mx)
+    codes = ora.morton(pos4, mn, mx, cubic=cubic)
     perm, srt = ora.sort_codes(codes)
     tree = ora.build_tree(srt, leaf)
     com = ora.tree_com(tree, pos4[perm])
@@ -137,7 +137,7 @@ def test_steps_kdk_vs_cpu_pipeline(nb, ora):
     op, ov, oa = pos.copy(), vel.copy(), np.zeros_like(pos)
     for _ in range(5):
         f, *_ = ora.bh_pipeline(pm4(op.astype(np.float32), m), EPS2, theta=0.5, leaf_size=8,
-                                group=64)
+                                group=64, cubic=True)
         ora.kdk_update_f64(op, ov, oa, f[:, :3].astype(np.float64), DT)
     assert np.abs(p - op).max() < 1e-6 * np.abs(op).max()
     assert rel_err(a, oa).max() < 1e-3
@@ -170,7 +170,7 @@ def test_c3_discs_properties(nb, ora):
     p4 = pm4(pos.astype(np.float32), m)
     mn, mx = ora.bbox(p4)
     np.testing.assert_array_equal(bbox[:3], mn)
-    ccodes = ora.morton(p4, mn, mx)
+    ccodes = ora.morton(p4, mn, mx, cubic=True)
     np.testing.assert_array_equal(codes, ccodes)
     cperm, srt = ora.sort_codes(ccodes)
     np.testing.assert_array_equal(perm, cperm)
@@ -185,3 +185,20 @@ def test_c3_discs_properties(nb, ora):
     e = rel_err(f[tgt], ref)
     print(f"C3 discs N=2^22 vs FP64 (512 targets): p50={np.median(e):.2e} p99={np.percentile(e, 99):.2e}")
     assert np.median(e) < 1e-2
+
+
+def test_axis_box_matches_reference_quantisation(nb, ora):
+    # NB_F_BH_AXIS_BOX: each axis over its own extent, the inputs of morton.glsl
+    n = 6000
+    pos, vel, m = nb.ic_discs(9, n)  # very flat: the axes' extents differ by ~20x
+    p4 = pm4(pos, m)
+    f, codes, perm, nodes, bbox, inter = gpu_run(nb, pos, vel, m, 8, flags=nb.F_BH_AXIS_BOX)
+    mn, mx, ccodes, cperm, tree, com = cpu_build(ora, p4, 8, cubic=False)
+    np.testing.assert_array_equal(codes, ccodes)
+    np.testing.assert_array_equal(perm, cperm)
+    np.testing.assert_array_equal(nodes["next"], tree["next"])
+    fc, cinter = ora.bh_forces_group(p4[cperm], cperm, mn, mx, tree, com, EPS2, group=64)
+    assert np.percentile(rel_err(f, fc), 99) < 1e-5
+    # the cubic default opens fewer cells on flat systems
+    _, _, _, _, _, inter_cubic = gpu_run(nb, pos, vel, m, 8)
+    assert inter_cubic < inter

@@ -271,3 +271,22 @@ def test_group_traversal_limits_and_accuracy(ora):
     # group of one body is the per-body traversal
     f1, *_, i1 = ora.bh_pipeline(pos, 40.0, theta=0.5, leaf_size=8, group=1)
     assert i1 == ib and rel_err(f1, fb).max() < 1e-6
+
+
+def test_cubic_box_is_the_reference_formula_on_a_cube(ora):
+    # with min at the origin, the cubic box is morton.glsl's formula with
+    # max = (E, E, E), E the largest extent
+    rng = np.random.default_rng(23)
+    pos = vec4(rng.uniform(0, 1, (2000, 3)) * np.array([100.0, 10.0, 1.0]))
+    pos[0, :3] = 0
+    pos[1, :3] = [100.0, 10.0, 1.0]
+    mn, mx = ora.bbox(pos)
+    cub = ora.morton(pos, mn, mx, cubic=True)
+    ref = ora.morton(pos, mn, np.float32([100.0, 100.0, 100.0]), cubic=False)
+    np.testing.assert_array_equal(cub, ref)
+    assert not np.array_equal(cub, ora.morton(pos, mn, mx, cubic=False))
+    # every cubic-box pipeline limit still holds
+    pos[:, 3] = 1.0
+    f0, *_, inter0 = ora.bh_pipeline(pos, 1.0, theta=1e-7, leaf_size=8, group=64, cubic=True)
+    ref_f = ora.direct_f64(pos[:, :3].astype(np.float64), np.ones(2000), 1.0)
+    assert inter0 == 2000 * 1999 and rel_err(f0, ref_f).max() < 1e-5
