@@ -34,6 +34,7 @@ INTEG_KDK_LAGGED = 0
 INTEG_EULER = 1
 F_NO_GRAPH = 0x1
 F_BH_PER_BODY = 0x4
+F_BH_AXIS_BOX = 0x8
 
 
 class NBodyError(RuntimeError):
