@@ -643,26 +643,69 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ pms
   }
 }
 
-// Wave-coherent traversal (the default): the 64 lanes of a wave hold 64
-// consecutive Morton-sorted bodies and walk the tree together. A cell is
-// accepted for the whole wave when side^2 < theta^2 dmin^2, dmin = distance from
-// the cell's COM to the wave's bounding box; otherwise the wave descends, and a
-// leaf that is not accepted is evaluated body by body by every lane. The node
-// index is wave-uniform, so node and leaf-body data arrive through the scalar
-// cache and the branches are scalar: no divergence. Per body this accepts a
-// subset of what the per-body test accepts (at least as accurate).
-__device__ __forceinline__ float wave_uniform(float v) {
-  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+// Wave traversal (the default). The 64 lanes of a wave hold 64 consecutive
+// Morton-sorted bodies and share every decision: a cell is accepted for the
+// whole wave when side^2 < theta^2 dmin^2, dmin = distance from the cell's COM
+// to the wave's bounding box; otherwise it is opened, and a leaf that is not
+// accepted is evaluated body by body by every lane. Per body this accepts a
+// subset of what the per-body test accepts (at least as accurate), and nothing
+// diverges. For latency, each wave keeps a stack of opened
+// nodes in LDS as (first child, level of the children << 27 | end of the body
+// range). One iteration pops up to 8 of them and lets 8 lanes per node examine
+// its (at most 8) children at once, so one memory round trip covers 64 nodes.
+// Accepted cells (their COM) and the bodies of unaccepted leaves are appended
+// to an interaction list in LDS; opened children are pushed. Whenever the list
+// fills up, and at the end, the wave streams it through the direct-sum inner
+// loop (LDS broadcast reads, 13 VALU per interaction).
+constexpr int kListWaves = 4;
+constexpr int kStackCap = 768;
+constexpr int kListCap = 256;
+
+template <bool kCount, bool kGuard>
+__device__ __forceinline__ void list_flush(float4* list, int lc, int lane, float4 me, float eps2,
+                                           float& ax, float& ay, float& az,
+                                           unsigned long long& cnt) {
+  if (lane < 4) list[lc + lane] = make_float4(0.f, 0.f, 0.f, 0.f);  // zero-mass pad
+  __builtin_amdgcn_wave_barrier();
+  for (int e = 0; e < lc; e += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const float4 q = list[e + u];  // same address on every lane: LDS broadcast
+      const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
+      const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+      const float inv = __builtin_amdgcn_rsqf(r2);
+      float s = (q.w * inv) * (inv * inv);
+      if (kGuard) s = r2 > 0.f ? s : 0.f;
+      ax = fmaf(dx, s, ax);
+      ay = fmaf(dy, s, ay);
+      az = fmaf(dz, s, az);
+      if (kCount) cnt += (q.w > 0.f && fmaf(dz, dz, fmaf(dy, dy, dx * dx)) > 0.f) ? 1 : 0;
+    }
+  }
+  __builtin_amdgcn_wave_barrier();
 }
 
-template <bool kCount>
-__global__ __launch_bounds__(256) void k_traverse_group(
+template <bool kCount, bool kGuard>
+__global__ __launch_bounds__(64 * kListWaves) void k_traverse_list(
     const float4* __restrict__ pms, const uint32_t* __restrict__ perm, int n,
     const float4* __restrict__ com, const int4* __restrict__ link,
     const int* __restrict__ level_off, int64_t max_nodes, const float* __restrict__ geom,
     float theta2, float eps2, double G, double* __restrict__ force,
     unsigned long long* __restrict__ inter, int* __restrict__ flags) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
+  __shared__ int2 s_stack[kListWaves][kStackCap];
+  __shared__ float4 s_list[kListWaves][kListCap + 4];
+  __shared__ int s_lvl[kMaxLevel + 2];
+  __shared__ float s_side2[kMaxLevel + 1];
+  if (threadIdx.x < kMaxLevel + 2) s_lvl[threadIdx.x] = (int)min((int64_t)level_off[threadIdx.x], max_nodes);
+  if (threadIdx.x <= kMaxLevel) s_side2[threadIdx.x] = geom[6 + threadIdx.x];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  int2* stack = s_stack[wave];
+  float4* list = s_list[wave];
+  const uint64_t lt = (uint64_t(1) << lane) - 1;
+
+  const int t = blockIdx.x * (64 * kListWaves) + threadIdx.x;
   const bool valid = t < n;
   const float4 me = pms[valid ? t : n - 1];
   float lx = me.x, ly = me.y, lz = me.z, hx = me.x, hy = me.y, hz = me.z;
@@ -675,70 +718,90 @@ __global__ __launch_bounds__(256) void k_traverse_group(
     hy = fmaxf(hy, __shfl_xor(hy, off));
     hz = fmaxf(hz, __shfl_xor(hz, off));
   }
-  lx = wave_uniform(lx);
-  ly = wave_uniform(ly);
-  lz = wave_uniform(lz);
-  hx = wave_uniform(hx);
-  hy = wave_uniform(hy);
-  hz = wave_uniform(hz);
-  const int total = (int)min((int64_t)level_off[kMaxLevel + 1], max_nodes);
   float ax = 0.f, ay = 0.f, az = 0.f;
   unsigned long long cnt = 0;
-  int i = 0;
-  for (int it = 0; i >= 0; ++it) {
-    if (i >= total || it > total) {  // a rope walk visits each node once
-      if (threadIdx.x % 64 == 0) atomicOr(flags, 4);
+  // a virtual parent whose only child is the root
+  if (lane == 0) stack[0] = make_int2(0, (int)((0u << 27) | (uint32_t)n));
+  __builtin_amdgcn_wave_barrier();
+  int sp = 1, lc = 0;
+  for (int iter = 0; sp > 0; ++iter) {
+    if (iter > s_lvl[kMaxLevel + 1]) {  // every node is examined at most once
+      if (lane == 0) atomicOr(flags, 4);
       break;
     }
-    const float4 c = com[i];
-    const int4 nd = link[i];
-    const float ex = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.f);
-    const float ey = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.f);
-    const float ez = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.f);
-    const float dmin2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
-    const bool accept = __builtin_amdgcn_readfirstlane(geom[6 + link_level(nd.w)] < theta2 * dmin2);
-    if (accept) {
-      if (c.w > 0.f) {
-        const float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
-        const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-        if (d2 > 0.f) {
-          const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
-          const float s = (c.w * inv) * (inv * inv);
-          ax = fmaf(dx, s, ax);
-          ay = fmaf(dy, s, ay);
-          az = fmaf(dz, s, az);
-          if (kCount) ++cnt;
+    const int k = min(sp, 8);
+    sp -= k;
+    const int q = lane >> 3, ci = lane & 7;
+    int child = -1;
+    int4 cl = make_int4(-1, -1, 0, 0);
+    float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (q < k) {
+      const int2 e = stack[sp + q];
+      const uint32_t w = (uint32_t)e.y;
+      const int clev = (int)(w >> 27);
+      const int pend = (int)(w & ((1u << 27) - 1));
+      const int cand = e.x + ci;
+      if (cand < s_lvl[clev + 1]) {
+        cl = link[cand];
+        if (cl.z < pend) {
+          child = cand;
+          c = com[cand];
         }
       }
-      i = nd.y;
-    } else if (nd.x < 0) {
-      // leaf bodies four at a time (one scalar load); bodies past the leaf's end
-      // (the next leaf, or the zero spare entries) get mass 0
-      const int e = min(nd.z + link_count(nd.w), n);
-      for (int b = max(nd.z, 0); b < e; b += 4) {
-        float4 q[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) q[u] = pms[b + u];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          const float w = b + u < e ? q[u].w : 0.f;
-          const float bx = q[u].x - me.x, by = q[u].y - me.y, bz = q[u].z - me.z;
-          const float b2 = fmaf(bz, bz, fmaf(by, by, bx * bx));
-          const float inv = __builtin_amdgcn_rsqf(b2 + eps2);
-          float s = (w * inv) * (inv * inv);
-          s = b2 > 0.f ? s : 0.f;
-          ax = fmaf(bx, s, ax);
-          ay = fmaf(by, s, ay);
-          az = fmaf(bz, s, az);
-          if (kCount) cnt += (b + u < e && b2 > 0.f) ? 1 : 0;
-        }
-      }
-      i = nd.y;
-    } else {
-      i = nd.x;
     }
-    i = __builtin_amdgcn_readfirstlane(i);
+    __builtin_amdgcn_wave_barrier();
+    bool accept = false, leaf = false, open = false;
+    if (child >= 0) {
+      const float ex = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.f);
+      const float ey = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.f);
+      const float ez = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.f);
+      const float dmin2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+      accept = s_side2[link_level(cl.w)] < theta2 * dmin2;
+      leaf = !accept && cl.x < 0;
+      open = !accept && cl.x >= 0;
+    }
+    // accepted cells -> list
+    const uint64_t ba = __ballot(accept && c.w > 0.f);
+    if (ba) {
+      if (lc + 64 > kListCap) {
+        list_flush<kCount, kGuard>(list, lc, lane, me, eps2, ax, ay, az, cnt);
+        lc = 0;
+      }
+      if (accept && c.w > 0.f) list[lc + __popcll(ba & lt)] = c;
+      lc += __popcll(ba);
+    }
+    // opened cells -> stack
+    const uint64_t bo = __ballot(open);
+    if (sp + __popcll(bo) > kStackCap) {
+      if (lane == 0) atomicOr(flags, 8);
+      break;
+    }
+    if (open) {
+      const uint32_t pend = (uint32_t)(cl.z + link_count(cl.w));
+      stack[sp + __popcll(bo & lt)] =
+          make_int2(cl.x, (int)(((uint32_t)(link_level(cl.w) + 1) << 27) | pend));
+    }
+    sp += __popcll(bo);
+    // leaves that are not accepted -> their bodies, 64 at a time
+    uint64_t bl = __ballot(leaf);
+    while (bl) {
+      const int src = __ffsll((unsigned long long)bl) - 1;
+      bl &= bl - 1;
+      const int ls = __builtin_amdgcn_readlane(cl.z, src);
+      const int lcount = __builtin_amdgcn_readlane(link_count(cl.w), src);
+      for (int off = 0; off < lcount; off += 64) {
+        const int m = min(64, lcount - off);
+        if (lc + 64 > kListCap) {
+          list_flush<kCount, kGuard>(list, lc, lane, me, eps2, ax, ay, az, cnt);
+          lc = 0;
+        }
+        if (lane < m) list[lc + lane] = pms[min(ls + off + lane, n - 1)];
+        lc += m;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
   }
+  list_flush<kCount, kGuard>(list, lc, lane, me, eps2, ax, ay, az, cnt);
   if (valid) {
     const uint32_t o = perm[t];
     if (o < (uint32_t)n) {
@@ -748,10 +811,10 @@ __global__ __launch_bounds__(256) void k_traverse_group(
     }
   }
   if (kCount) {
-    __shared__ unsigned long long red[256];
+    __shared__ unsigned long long red[64 * kListWaves];
     red[threadIdx.x] = valid ? cnt : 0ull;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
+    for (int s = 64 * kListWaves / 2; s > 0; s >>= 1) {
       if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
       __syncthreads();
     }
@@ -902,10 +965,20 @@ hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions) {
 #define NB_TRAVERSE_ARGS                                                                       \
   b.pms, b.vals[0], n, b.node_com, b.node_link, b.level_off, b.max_nodes, b.geom, c->theta2, \
       c->cfg.eps2, c->G, c->force, b.inter, c->flags
-  if (group && count_interactions)
-    hipLaunchKernelGGL((k_traverse_group<true>), dim3(g256), dim3(256), 0, c->stream, NB_TRAVERSE_ARGS);
+  const bool guard = c->guard;
+  const int gl = (n + 64 * kListWaves - 1) / (64 * kListWaves);
+  if (group && count_interactions && guard)
+    hipLaunchKernelGGL((k_traverse_list<true, true>), dim3(gl), dim3(64 * kListWaves), 0, c->stream,
+                       NB_TRAVERSE_ARGS);
+  else if (group && count_interactions)
+    hipLaunchKernelGGL((k_traverse_list<true, false>), dim3(gl), dim3(64 * kListWaves), 0, c->stream,
+                       NB_TRAVERSE_ARGS);
+  else if (group && guard)
+    hipLaunchKernelGGL((k_traverse_list<false, true>), dim3(gl), dim3(64 * kListWaves), 0, c->stream,
+                       NB_TRAVERSE_ARGS);
   else if (group)
-    hipLaunchKernelGGL((k_traverse_group<false>), dim3(g256), dim3(256), 0, c->stream, NB_TRAVERSE_ARGS);
+    hipLaunchKernelGGL((k_traverse_list<false, false>), dim3(gl), dim3(64 * kListWaves), 0, c->stream,
+                       NB_TRAVERSE_ARGS);
   else if (count_interactions)
     hipLaunchKernelGGL((k_traverse<true>), dim3(g256), dim3(256), 0, c->stream, NB_TRAVERSE_ARGS);
   else

@@ -59,6 +59,7 @@ int check_flags(nb_ctx* c) {
   if (h & 1) return nb_fail(NB_E_NONFINITE, "non-finite position or velocity in the particle state");
   if (h & 2) return nb_fail(NB_E_TREE_OVERFLOW, "octree node pool exhausted");
   if (h & 4) return nb_fail(NB_E_TREE_OVERFLOW, "octree traversal left the node range");
+  if (h & 8) return nb_fail(NB_E_TREE_OVERFLOW, "octree traversal stack overflow");
   return NB_OK;
 }
 
