@@ -287,6 +287,8 @@ def main():
                     help="also run the serving stream for this many seconds per mode")
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (built-in shapes)")
+    ap.add_argument("--only", choices=["direct", "bh", "batch"], default=None,
+                    help="run one workload alone (for profiling); prints its sub-object")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -302,6 +304,14 @@ def main():
     import nbody_amd as nb
 
     log(nb.device_info())
+    if args.only == "bh":
+        print(json.dumps(bench_bh(nb, args.bh_steps, 2)), flush=True)
+        return
+    if args.only == "batch":
+        print(json.dumps(bench_batch(nb, args.batch_steps, 2, args.batch_energy)), flush=True)
+        return
+    if args.only == "direct":
+        args.no_bh = args.no_batch = args.no_cpu = True
     n = args.n
     tuning = None
     if not args.no_tune:

@@ -649,40 +649,55 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ pms
 // to the wave's bounding box; otherwise it is opened, and a leaf that is not
 // accepted is evaluated body by body by every lane. Per body this accepts a
 // subset of what the per-body test accepts (at least as accurate), and nothing
-// diverges. For latency, each wave keeps a stack of opened
-// nodes in LDS as (first child, level of the children << 27 | end of the body
-// range). One iteration pops up to 8 of them and lets 8 lanes per node examine
-// its (at most 8) children at once, so one memory round trip covers 64 nodes.
-// Accepted cells (their COM) and the bodies of unaccepted leaves are appended
-// to an interaction list in LDS; opened children are pushed. Whenever the list
-// fills up, and at the end, the wave streams it through the direct-sum inner
-// loop (LDS broadcast reads, 13 VALU per interaction).
+// diverges. For latency, each wave keeps a stack of opened nodes in LDS as
+// (first child, level of the children << 27 | end of the body range). One
+// iteration pops up to 8 of them and lets 8 lanes per node examine its (at most
+// 8) children at once, so one memory round trip covers 64 nodes. The walk only
+// appends indices to an interaction list in LDS (accepted cells, flagged, and
+// the bodies of unaccepted leaves) and pushes opened children; no source data is
+// read while walking. Whenever the list fills up, and at the end, the wave
+// gathers the sources 64 at a time into LDS and streams them through the
+// direct-sum inner loop (LDS broadcast reads, 13 VALU per interaction).
 constexpr int kListWaves = 4;
 constexpr int kStackCap = 768;
-constexpr int kListCap = 256;
+constexpr int kListCap = 512;                 // source indices per wave
+constexpr uint32_t kCellBit = 0x80000000u;    // list entry refers to a cell, not a body
 
+// Stream the list through the direct-sum inner loop: 64 entries at a time,
+// each lane gathers one source (cell COM or body) into the LDS staging area,
+// then every lane reads the 64 staged sources by broadcast.
 template <bool kCount, bool kGuard>
-__device__ __forceinline__ void list_flush(float4* list, int lc, int lane, float4 me, float eps2,
+__device__ __forceinline__ void list_flush(const uint32_t* list, int lc, float4* stage, int lane,
+                                           const float4* __restrict__ pms,
+                                           const float4* __restrict__ com, float4 me, float eps2,
                                            float& ax, float& ay, float& az,
                                            unsigned long long& cnt) {
-  if (lane < 4) list[lc + lane] = make_float4(0.f, 0.f, 0.f, 0.f);  // zero-mass pad
-  __builtin_amdgcn_wave_barrier();
-  for (int e = 0; e < lc; e += 4) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const float4 q = list[e + u];  // same address on every lane: LDS broadcast
-      const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
-      const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
-      const float inv = __builtin_amdgcn_rsqf(r2);
-      float s = (q.w * inv) * (inv * inv);
-      if (kGuard) s = r2 > 0.f ? s : 0.f;
-      ax = fmaf(dx, s, ax);
-      ay = fmaf(dy, s, ay);
-      az = fmaf(dz, s, az);
-      if (kCount) cnt += (q.w > 0.f && fmaf(dz, dz, fmaf(dy, dy, dx * dx)) > 0.f) ? 1 : 0;
+  for (int e = 0; e < lc; e += 64) {
+    const int m = min(64, lc - e);
+    float4 q = make_float4(0.f, 0.f, 0.f, 0.f);  // zero-mass pad
+    if (lane < m) {
+      const uint32_t id = list[e + lane];
+      q = (id & kCellBit) ? com[id & ~kCellBit] : pms[id];
     }
+    stage[lane] = q;
+    __builtin_amdgcn_wave_barrier();
+    for (int j = 0; j < m; j += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const float4 src = stage[j + u];  // same address on every lane: LDS broadcast
+        const float dx = src.x - me.x, dy = src.y - me.y, dz = src.z - me.z;
+        const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
+        const float inv = __builtin_amdgcn_rsqf(r2);
+        float s = (src.w * inv) * (inv * inv);
+        if (kGuard) s = r2 > 0.f ? s : 0.f;
+        ax = fmaf(dx, s, ax);
+        ay = fmaf(dy, s, ay);
+        az = fmaf(dz, s, az);
+        if (kCount) cnt += (src.w > 0.f && fmaf(dz, dz, fmaf(dy, dy, dx * dx)) > 0.f) ? 1 : 0;
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
   }
-  __builtin_amdgcn_wave_barrier();
 }
 
 template <bool kCount, bool kGuard>
@@ -693,7 +708,8 @@ __global__ __launch_bounds__(64 * kListWaves) void k_traverse_list(
     float theta2, float eps2, double G, double* __restrict__ force,
     unsigned long long* __restrict__ inter, int* __restrict__ flags) {
   __shared__ int2 s_stack[kListWaves][kStackCap];
-  __shared__ float4 s_list[kListWaves][kListCap + 4];
+  __shared__ uint32_t s_list[kListWaves][kListCap];
+  __shared__ float4 s_stage[kListWaves][64 + 4];
   __shared__ int s_lvl[kMaxLevel + 2];
   __shared__ float s_side2[kMaxLevel + 1];
   if (threadIdx.x < kMaxLevel + 2) s_lvl[threadIdx.x] = (int)min((int64_t)level_off[threadIdx.x], max_nodes);
@@ -702,7 +718,8 @@ __global__ __launch_bounds__(64 * kListWaves) void k_traverse_list(
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   int2* stack = s_stack[wave];
-  float4* list = s_list[wave];
+  uint32_t* list = s_list[wave];
+  float4* stage = s_stage[wave];
   const uint64_t lt = (uint64_t(1) << lane) - 1;
 
   const int t = blockIdx.x * (64 * kListWaves) + threadIdx.x;
@@ -720,8 +737,13 @@ __global__ __launch_bounds__(64 * kListWaves) void k_traverse_list(
   }
   float ax = 0.f, ay = 0.f, az = 0.f;
   unsigned long long cnt = 0;
+#define NB_FLUSH()                                                                               \
+  do {                                                                                           \
+    list_flush<kCount, kGuard>(list, lc, stage, lane, pms, com, me, eps2, ax, ay, az, cnt);     \
+    lc = 0;                                                                                      \
+  } while (0)
   // a virtual parent whose only child is the root
-  if (lane == 0) stack[0] = make_int2(0, (int)((0u << 27) | (uint32_t)n));
+  if (lane == 0) stack[0] = make_int2(0, (int)(uint32_t)n);
   __builtin_amdgcn_wave_barrier();
   int sp = 1, lc = 0;
   for (int iter = 0; sp > 0; ++iter) {
@@ -760,14 +782,12 @@ __global__ __launch_bounds__(64 * kListWaves) void k_traverse_list(
       leaf = !accept && cl.x < 0;
       open = !accept && cl.x >= 0;
     }
-    // accepted cells -> list
-    const uint64_t ba = __ballot(accept && c.w > 0.f);
+    // accepted cells with mass -> list
+    const bool acc = accept && c.w > 0.f;
+    const uint64_t ba = __ballot(acc);
     if (ba) {
-      if (lc + 64 > kListCap) {
-        list_flush<kCount, kGuard>(list, lc, lane, me, eps2, ax, ay, az, cnt);
-        lc = 0;
-      }
-      if (accept && c.w > 0.f) list[lc + __popcll(ba & lt)] = c;
+      if (lc + 64 > kListCap) NB_FLUSH();
+      if (acc) list[lc + __popcll(ba & lt)] = (uint32_t)child | kCellBit;
       lc += __popcll(ba);
     }
     // opened cells -> stack
@@ -782,7 +802,7 @@ __global__ __launch_bounds__(64 * kListWaves) void k_traverse_list(
           make_int2(cl.x, (int)(((uint32_t)(link_level(cl.w) + 1) << 27) | pend));
     }
     sp += __popcll(bo);
-    // leaves that are not accepted -> their bodies, 64 at a time
+    // leaves that are not accepted -> the indices of their bodies
     uint64_t bl = __ballot(leaf);
     while (bl) {
       const int src = __ffsll((unsigned long long)bl) - 1;
@@ -791,17 +811,15 @@ __global__ __launch_bounds__(64 * kListWaves) void k_traverse_list(
       const int lcount = __builtin_amdgcn_readlane(link_count(cl.w), src);
       for (int off = 0; off < lcount; off += 64) {
         const int m = min(64, lcount - off);
-        if (lc + 64 > kListCap) {
-          list_flush<kCount, kGuard>(list, lc, lane, me, eps2, ax, ay, az, cnt);
-          lc = 0;
-        }
-        if (lane < m) list[lc + lane] = pms[min(ls + off + lane, n - 1)];
+        if (lc + 64 > kListCap) NB_FLUSH();
+        if (lane < m) list[lc + lane] = (uint32_t)min(ls + off + lane, n - 1);
         lc += m;
       }
     }
     __builtin_amdgcn_wave_barrier();
   }
-  list_flush<kCount, kGuard>(list, lc, lane, me, eps2, ax, ay, az, cnt);
+  NB_FLUSH();
+#undef NB_FLUSH
   if (valid) {
     const uint32_t o = perm[t];
     if (o < (uint32_t)n) {
