@@ -49,6 +49,15 @@ constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 elements per block
 constexpr int kScanTile = 2048;             // elements per block in the scans
 constexpr int kScanThreads = 256;
 constexpr int kSpare = 4;  // zero-mass bodies after the sorted body array
+// Traversal groups: each wave of the traversal takes one "wave range" of at
+// most kWaveBodies consecutive sorted bodies. Ranges are cut along the tree: the
+// children of a node holding more than kWaveBodies bodies are packed in order
+// into ranges of at most kWaveBodies bodies (a child that itself holds more is
+// handled at its own level; a level-10 leaf that holds more is split into
+// chunks), so every range lies inside one parent cell and its bounding box
+// stays tight. (With leaf_size <= kWaveBodies only level-10 leaves can hold
+// more than kWaveBodies bodies.)
+constexpr int kWaveBodies = 64;
 
 // ---- 1. bounding box --------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bbox_partial(const float4* __restrict__ pm, int n,
@@ -449,9 +458,14 @@ __global__ __launch_bounds__(256) void k_gather(const float4* __restrict__ pm,
 __device__ __forceinline__ int link_count(int w) { return w >> 4; }
 __device__ __forceinline__ int link_level(int w) { return w & 15; }
 
-__global__ void k_tree_init(int n, int4* __restrict__ link, int* __restrict__ level_off) {
+__global__ void k_tree_init(int n, int4* __restrict__ link, int* __restrict__ level_off,
+                            int2* __restrict__ waves, int* __restrict__ wctl) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   link[0] = make_int4(-1, -1, 0, n << 4);
+  // wave ranges: a system of at most kWaveBodies bodies is one range
+  wctl[0] = n <= kWaveBodies ? 1 : 0;  // range count
+  wctl[1] = 0;                         // traversal ticket
+  if (n <= kWaveBodies) waves[0] = make_int2(0, n);
   level_off[0] = 0;
   level_off[1] = 1;
   for (int l = 2; l <= kMaxLevel + 1; ++l) level_off[l] = 1;
@@ -498,10 +512,16 @@ __global__ __launch_bounds__(256) void k_tree_emit(const uint32_t* __restrict__ 
                                                    const int* __restrict__ level_off, int l,
                                                    int leaf, int cap,
                                                    const int* __restrict__ child_off,
-                                                   int64_t max_nodes, int* __restrict__ flags) {
+                                                   int64_t max_nodes, int* __restrict__ flags,
+                                                   int2* __restrict__ waves,
+                                                   int* __restrict__ wctl) {
   const int lo = level_off[l];
   const int hi = min(level_off[l + 1], lo + cap);
   const int next_lo = level_off[l + 1];
+  auto emit = [&](int start, int count) {
+    const int slot = atomicAdd(&wctl[0], 1);
+    if (slot < n) waves[slot] = make_int2(start, count);  // at most n ranges exist
+  };
   const int shift = 27 - 3 * l;
   for (int k = lo + blockIdx.x * 256 + threadIdx.x; k < hi; k += gridDim.x * 256) {
     int4 nd = link[k];
@@ -530,6 +550,26 @@ __global__ __launch_bounds__(256) void k_tree_emit(const uint32_t* __restrict__ 
       link[idx] = make_int4(-1, nxt, starts[q], ((ends[q] - starts[q]) << 4) | (l + 1));
     }
     link[k].x = first;
+    if (cnt > kWaveBodies) {  // pack the children into wave ranges
+      int run_s = 0, run_n = 0;
+      for (int q = 0; q < nc; ++q) {
+        const int cn = ends[q] - starts[q];
+        if (cn > kWaveBodies) {
+          if (run_n) emit(run_s, run_n);
+          run_n = 0;
+          if (l + 1 == kMaxLevel || cn <= leaf)  // a leaf: chunks (internal: its own level)
+            for (int o = 0; o < cn; o += kWaveBodies) emit(starts[q] + o, min(kWaveBodies, cn - o));
+        } else {
+          if (run_n + cn > kWaveBodies) {
+            emit(run_s, run_n);
+            run_n = 0;
+          }
+          if (run_n == 0) run_s = starts[q];
+          run_n += cn;
+        }
+      }
+      if (run_n) emit(run_s, run_n);
+    }
   }
 }
 
@@ -700,31 +740,24 @@ __device__ __forceinline__ void list_flush(const uint32_t* list, int lc, float4*
   }
 }
 
-template <bool kCount, bool kGuard>
-__global__ __launch_bounds__(64 * kListWaves) void k_traverse_list(
-    const float4* __restrict__ pms, const uint32_t* __restrict__ perm, int n,
-    const float4* __restrict__ com, const int4* __restrict__ link,
-    const int* __restrict__ level_off, int64_t max_nodes, const float* __restrict__ geom,
-    float theta2, float eps2, double G, double* __restrict__ force,
-    unsigned long long* __restrict__ inter, int* __restrict__ flags) {
-  __shared__ int2 s_stack[kListWaves][kStackCap];
-  __shared__ uint32_t s_list[kListWaves][kListCap];
-  __shared__ float4 s_stage[kListWaves][64 + 4];
-  __shared__ int s_lvl[kMaxLevel + 2];
-  __shared__ float s_side2[kMaxLevel + 1];
-  if (threadIdx.x < kMaxLevel + 2) s_lvl[threadIdx.x] = (int)min((int64_t)level_off[threadIdx.x], max_nodes);
-  if (threadIdx.x <= kMaxLevel) s_side2[threadIdx.x] = geom[6 + threadIdx.x];
-  __syncthreads();
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  int2* stack = s_stack[wave];
-  uint32_t* list = s_list[wave];
-  float4* stage = s_stage[wave];
-  const uint64_t lt = (uint64_t(1) << lane) - 1;
+// Shared LDS of one traversing wave.
+struct WaveLds {
+  int2* stack;
+  uint32_t* list;
+  float4* stage;
+  const int* lvl;      // first node of each level (kMaxLevel + 2 entries)
+  const float* side2;  // cell side^2 per level
+};
 
-  const int t = blockIdx.x * (64 * kListWaves) + threadIdx.x;
-  const bool valid = t < n;
-  const float4 me = pms[valid ? t : n - 1];
+// Walk the tree for the bodies of one wave range (lanes past the range hold a
+// copy of a valid body) and accumulate their accelerations.
+template <bool kCount, bool kGuard>
+__device__ void traverse_range(const WaveLds& L, int lane, float4 me, int n,
+                               const float4* __restrict__ pms, const float4* __restrict__ com,
+                               const int4* __restrict__ link, float theta2, float eps2,
+                               int* __restrict__ flags, float& ax, float& ay, float& az,
+                               unsigned long long& cnt) {
+  const uint64_t lt = (uint64_t(1) << lane) - 1;
   float lx = me.x, ly = me.y, lz = me.z, hx = me.x, hy = me.y, hz = me.z;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -735,19 +768,19 @@ __global__ __launch_bounds__(64 * kListWaves) void k_traverse_list(
     hy = fmaxf(hy, __shfl_xor(hy, off));
     hz = fmaxf(hz, __shfl_xor(hz, off));
   }
-  float ax = 0.f, ay = 0.f, az = 0.f;
-  unsigned long long cnt = 0;
-#define NB_FLUSH()                                                                               \
-  do {                                                                                           \
-    list_flush<kCount, kGuard>(list, lc, stage, lane, pms, com, me, eps2, ax, ay, az, cnt);     \
-    lc = 0;                                                                                      \
+  int2* stack = L.stack;
+  uint32_t* list = L.list;
+#define NB_FLUSH()                                                                              \
+  do {                                                                                          \
+    list_flush<kCount, kGuard>(list, lc, L.stage, lane, pms, com, me, eps2, ax, ay, az, cnt);  \
+    lc = 0;                                                                                     \
   } while (0)
   // a virtual parent whose only child is the root
   if (lane == 0) stack[0] = make_int2(0, (int)(uint32_t)n);
   __builtin_amdgcn_wave_barrier();
   int sp = 1, lc = 0;
   for (int iter = 0; sp > 0; ++iter) {
-    if (iter > s_lvl[kMaxLevel + 1]) {  // every node is examined at most once
+    if (iter > L.lvl[kMaxLevel + 1]) {  // every node is examined at most once
       if (lane == 0) atomicOr(flags, 4);
       break;
     }
@@ -763,7 +796,7 @@ __global__ __launch_bounds__(64 * kListWaves) void k_traverse_list(
       const int clev = (int)(w >> 27);
       const int pend = (int)(w & ((1u << 27) - 1));
       const int cand = e.x + ci;
-      if (cand < s_lvl[clev + 1]) {
+      if (cand < L.lvl[clev + 1]) {
         cl = link[cand];
         if (cl.z < pend) {
           child = cand;
@@ -778,7 +811,7 @@ __global__ __launch_bounds__(64 * kListWaves) void k_traverse_list(
       const float ey = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.f);
       const float ez = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.f);
       const float dmin2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
-      accept = s_side2[link_level(cl.w)] < theta2 * dmin2;
+      accept = L.side2[link_level(cl.w)] < theta2 * dmin2;
       leaf = !accept && cl.x < 0;
       open = !accept && cl.x >= 0;
     }
@@ -820,17 +853,58 @@ __global__ __launch_bounds__(64 * kListWaves) void k_traverse_list(
   }
   NB_FLUSH();
 #undef NB_FLUSH
-  if (valid) {
-    const uint32_t o = perm[t];
-    if (o < (uint32_t)n) {
-      force[o] = G * (double)ax;
-      force[n + o] = G * (double)ay;
-      force[2 * n + o] = G * (double)az;
+}
+
+// Persistent: every wave takes wave ranges from a ticket until none is left,
+// so the cost of uneven ranges balances dynamically.
+template <bool kCount, bool kGuard>
+__global__ __launch_bounds__(64 * kListWaves) void k_traverse_list(
+    const float4* __restrict__ pms, const uint32_t* __restrict__ perm, int n,
+    const float4* __restrict__ com, const int4* __restrict__ link,
+    const int* __restrict__ level_off, int64_t max_nodes, const float* __restrict__ geom,
+    float theta2, float eps2, double G, double* __restrict__ force,
+    unsigned long long* __restrict__ inter, int* __restrict__ flags,
+    const int2* __restrict__ waves, int* __restrict__ wctl) {
+  __shared__ int2 s_stack[kListWaves][kStackCap];
+  __shared__ uint32_t s_list[kListWaves][kListCap];
+  __shared__ float4 s_stage[kListWaves][64];
+  __shared__ int s_lvl[kMaxLevel + 2];
+  __shared__ float s_side2[kMaxLevel + 1];
+  if (threadIdx.x < kMaxLevel + 2)
+    s_lvl[threadIdx.x] = (int)min((int64_t)level_off[threadIdx.x], max_nodes);
+  if (threadIdx.x <= kMaxLevel) s_side2[threadIdx.x] = geom[6 + threadIdx.x];
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const WaveLds L{s_stack[wave], s_list[wave], s_stage[wave], s_lvl, s_side2};
+  const int nranges = min(wctl[0], n);
+  unsigned long long cnt = 0;
+  for (;;) {
+    int w = 0;
+    if (lane == 0) w = atomicAdd(&wctl[1], 1);
+    w = __shfl(w, 0);
+    if (w >= nranges) break;
+    const int2 range = waves[w];
+    const bool valid = lane < range.y;
+    const int t = min(range.x + (valid ? lane : 0), n - 1);
+    const float4 me = pms[t];
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    unsigned long long c = 0;
+    traverse_range<kCount, kGuard>(L, lane, me, n, pms, com, link, theta2, eps2, flags, ax, ay,
+                                   az, c);
+    if (valid) {
+      const uint32_t o = perm[t];
+      if (o < (uint32_t)n) {
+        force[o] = G * (double)ax;
+        force[n + o] = G * (double)ay;
+        force[2 * n + o] = G * (double)az;
+      }
+      cnt += c;
     }
   }
   if (kCount) {
     __shared__ unsigned long long red[64 * kListWaves];
-    red[threadIdx.x] = valid ? cnt : 0ull;
+    red[threadIdx.x] = cnt;
     __syncthreads();
     for (int s = 64 * kListWaves / 2; s > 0; s >>= 1) {
       if (threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
@@ -876,6 +950,12 @@ int nb_bh_alloc(nb_ctx* c) {
   }
   b.max_nodes = bound;
   b.bbox_blocks = kBboxBlocks;
+  {
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    b.persist_blocks = 4 * cus;  // the traversal's LDS allows four blocks per CU
+  }
   b.sort_blocks = (int)((n + kSortTile - 1) / kSortTile);
   const int64_t hist = 256 * (int64_t)b.sort_blocks;
   const int64_t scan_parts = (std::max<int64_t>(hist, n) + kScanTile - 1) / kScanTile + 2;
@@ -895,6 +975,7 @@ int nb_bh_alloc(nb_ctx* c) {
       A((void**)&b.child_cnt, sizeof(int) * n) || A((void**)&b.child_off, sizeof(int) * n) ||
       A((void**)&b.iscan_part, sizeof(int) * scan_parts) ||
       A((void**)&b.counters, sizeof(int) * 16) ||
+      A((void**)&b.waves, sizeof(int2) * n) ||
       A((void**)&b.inter, sizeof(unsigned long long) * 2))
     return nb_fail(NB_E_OOM, "nb_create: device allocation failed (octree)");
   // the batched leaf loop may read up to kSpare bodies past the end: zero mass
@@ -908,7 +989,7 @@ void nb_bh_free(nb_ctx* c) {
   nb_bh_state& b = c->bh;
   void* ps[] = {b.bbox_part, b.geom,     b.keys[0],  b.keys[1],   b.vals[0],    b.vals[1],
                 b.codes,     b.digit_hist, b.pms,    b.prefix,    b.scan_part,  b.node_com,
-                b.node_link, b.level_off, b.child_cnt, b.child_off, b.iscan_part, b.counters,
+                b.node_link, b.level_off, b.child_cnt, b.child_off, b.iscan_part, b.counters, b.waves,
                 b.inter};
   for (void* p : ps)
     if (p) (void)hipFree(p);
@@ -952,7 +1033,8 @@ hipError_t nb_bh_build(nb_ctx* c) {
   hipLaunchKernelGGL(k_mscan_final, dim3(mblocks), dim3(kScanThreads), 0, s, b.pms, n, b.scan_part,
                      b.prefix);
   // 6. octree, level by level
-  hipLaunchKernelGGL(k_tree_init, dim3(1), dim3(64), 0, s, n, b.node_link, b.level_off);
+  hipLaunchKernelGGL(k_tree_init, dim3(1), dim3(64), 0, s, n, b.node_link, b.level_off, b.waves,
+                     b.counters);
   const int leaf = c->cfg.leaf_size;
   for (int l = 0; l < kMaxLevel; ++l) {
     const int bound = level_bound(n, l);
@@ -963,7 +1045,8 @@ hipError_t nb_bh_build(nb_ctx* c) {
                          b.level_off, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_tree_emit, dim3(grid), dim3(256), 0, s, codes_sorted, n, b.node_link,
-                       b.level_off, l, leaf, bound, b.child_off, b.max_nodes, c->flags);
+                       b.level_off, l, leaf, bound, b.child_off, b.max_nodes, c->flags, b.waves,
+                       b.counters);
   }
   // 7. centre of mass
   const int cgrid = (int)std::max<int64_t>(1, std::min<int64_t>((b.max_nodes + 255) / 256, 8192));
@@ -984,19 +1067,21 @@ hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions) {
   b.pms, b.vals[0], n, b.node_com, b.node_link, b.level_off, b.max_nodes, b.geom, c->theta2, \
       c->cfg.eps2, c->G, c->force, b.inter, c->flags
   const bool guard = c->guard;
-  const int gl = (n + 64 * kListWaves - 1) / (64 * kListWaves);
+  const int gl = b.persist_blocks;  // persistent blocks pulling wave ranges
+#define NB_LIST_ARGS NB_TRAVERSE_ARGS, b.waves, b.counters
   if (group && count_interactions && guard)
     hipLaunchKernelGGL((k_traverse_list<true, true>), dim3(gl), dim3(64 * kListWaves), 0, c->stream,
-                       NB_TRAVERSE_ARGS);
+                       NB_LIST_ARGS);
   else if (group && count_interactions)
     hipLaunchKernelGGL((k_traverse_list<true, false>), dim3(gl), dim3(64 * kListWaves), 0, c->stream,
-                       NB_TRAVERSE_ARGS);
+                       NB_LIST_ARGS);
   else if (group && guard)
     hipLaunchKernelGGL((k_traverse_list<false, true>), dim3(gl), dim3(64 * kListWaves), 0, c->stream,
-                       NB_TRAVERSE_ARGS);
+                       NB_LIST_ARGS);
   else if (group)
     hipLaunchKernelGGL((k_traverse_list<false, false>), dim3(gl), dim3(64 * kListWaves), 0, c->stream,
-                       NB_TRAVERSE_ARGS);
+                       NB_LIST_ARGS);
+#undef NB_LIST_ARGS
   else if (count_interactions)
     hipLaunchKernelGGL((k_traverse<true>), dim3(g256), dim3(256), 0, c->stream, NB_TRAVERSE_ARGS);
   else

@@ -50,7 +50,9 @@ struct nb_bh_state {
   int* child_cnt = nullptr;     // per node of the current level: non-empty children
   int* child_off = nullptr;     // exclusive scan of child_cnt
   int* iscan_part = nullptr;    // int scan block partials
-  int* counters = nullptr;      // misc device counters
+  int* counters = nullptr;      // [0] wave ranges, [1] traversal ticket
+  int2* waves = nullptr;        // wave ranges (first sorted body, count <= 64)
+  int persist_blocks = 0;       // traversal workgroups (persistent)
   unsigned long long* inter = nullptr;  // interaction counter (diagnostic traversal)
 };
 

@@ -874,13 +874,46 @@ int64_t or_bh_forces(int64_t n, const float* pos4s, const uint32_t* perm, const 
   return total;
 }
 
+int64_t or_bh_wave_ranges(int64_t n, int64_t nodes, const int32_t* first_child,
+                          const int32_t* start, const int32_t* nbody, const int32_t* level,
+                          int cap, int32_t* range_start, int32_t* range_count) {
+  int64_t r = 0;
+  if (n <= cap) {
+    range_start[0] = 0;
+    range_count[0] = (int32_t)n;
+    return 1;
+  }
+  for (int64_t k = 0; k < nodes; ++k) {
+    if (first_child[k] < 0 || nbody[k] <= cap) continue;
+    const int64_t end = (int64_t)start[k] + nbody[k];
+    int32_t run_s = 0, run_n = 0;
+    for (int64_t c = first_child[k]; c < nodes && level[c] == level[k] + 1 && start[c] < end; ++c) {
+      const int32_t cn = nbody[c];
+      if (cn > cap) {
+        if (run_n) { range_start[r] = run_s; range_count[r++] = run_n; }
+        run_n = 0;
+        if (first_child[c] < 0)  /* a leaf: chunks; an internal child packs its own children */
+          for (int32_t o = 0; o < cn; o += cap) {
+            range_start[r] = start[c] + o;
+            range_count[r++] = cn - o < cap ? cn - o : cap;
+          }
+      } else {
+        if (run_n + cn > cap) { range_start[r] = run_s; range_count[r++] = run_n; run_n = 0; }
+        if (run_n == 0) run_s = start[c];
+        run_n += cn;
+      }
+    }
+    if (run_n) { range_start[r] = run_s; range_count[r++] = run_n; }
+  }
+  return r;
+}
+
 int64_t or_bh_forces_group(int64_t n, const float* pos4s, const uint32_t* perm,
                            const float bmin[3], const float bmax[3], int64_t nodes,
                            const int32_t* first_child, const int32_t* next, const int32_t* start,
                            const int32_t* nbody, const int32_t* level, const float* com4,
                            float eps2, float G, float theta, int group, float* force4,
                            int nthreads) {
-  (void)nodes;
   if (group < 1 || group > 64) return -1;
   float side2[11];
   const float s0 = root_side(bmin, bmax);
@@ -889,11 +922,13 @@ int64_t or_bh_forces_group(int64_t n, const float* pos4s, const uint32_t* perm,
     side2[l] = s * s;
   }
   const float theta2 = theta * theta;
-  const int64_t ngroups = (n + group - 1) / group;
+  int32_t* rs = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
+  int32_t* rc = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
+  const int64_t ngroups = or_bh_wave_ranges(n, nodes, first_child, start, nbody, level, group, rs, rc);
   int64_t total = 0;
 #pragma omp parallel for schedule(dynamic, 16) reduction(+ : total) num_threads(clamp_threads(nthreads))
   for (int64_t g = 0; g < ngroups; ++g) {
-    const int64_t t0 = g * group, t1 = (t0 + group < n) ? t0 + group : n;
+    const int64_t t0 = rs[g], t1 = (int64_t)rs[g] + rc[g];
     float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
     for (int64_t t = t0; t < t1; ++t)
       for (int a = 0; a < 3; ++a) {
@@ -956,6 +991,8 @@ int64_t or_bh_forces_group(int64_t n, const float* pos4s, const uint32_t* perm,
     }
     total += cnt;
   }
+  free(rs);
+  free(rc);
   return total;
 }
 

@@ -127,12 +127,21 @@ int64_t or_bh_forces(int64_t n, const float* pos4s, const uint32_t* perm, const 
                      const int32_t* next, const int32_t* start, const int32_t* nbody,
                      const int32_t* level, const float* com4, float eps2, float G, float theta,
                      float* force4, int nthreads);
-/* Group traversal (the engine's default on the GPU): bodies are taken in
- * groups of `group` consecutive sorted bodies; a cell is accepted for the whole
- * group when s_l^2 < theta^2 dmin^2, dmin = distance from the cell's COM to the
- * group's bounding box, else the group descends; a leaf that is not accepted is
- * evaluated body by body for every member. Per body this accepts a subset of the
- * cells the per-body test accepts, so it is at least as accurate. */
+/* Wave ranges of the engine's traversal: for every node holding more than cap
+ * bodies, its children are packed in order into ranges of at most cap bodies; a
+ * child holding more packs its own children at its level if it is internal, and
+ * is cut into chunks of cap if it is a leaf. A system of at most cap bodies is
+ * one range. Every body lies in exactly one range. Returns the number of ranges. */
+int64_t or_bh_wave_ranges(int64_t n, int64_t nodes, const int32_t* first_child,
+                          const int32_t* start, const int32_t* nbody, const int32_t* level,
+                          int cap, int32_t* range_start, int32_t* range_count);
+/* Group traversal (the engine's default on the GPU): the bodies of one wave
+ * range share every decision: a cell is accepted for the group when
+ * s_l^2 < theta^2 dmin^2, dmin = distance from the cell's COM to the range's
+ * bounding box, else the group descends; a leaf that is not accepted is
+ * evaluated body by body for every member. Per body this accepts a subset of
+ * the cells the per-body test accepts, so it is at least as accurate. group is
+ * the range capacity (64 on the GPU; 1 gives the per-body traversal). */
 int64_t or_bh_forces_group(int64_t n, const float* pos4s, const uint32_t* perm,
                            const float bmin[3], const float bmax[3], int64_t nodes,
                            const int32_t* first_child, const int32_t* next, const int32_t* start,

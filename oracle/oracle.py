@@ -39,6 +39,7 @@ _SIGS = {
                           C.c_float, _P, C.c_int]),
     "or_bh_forces_group": (_I, [_I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, C.c_float, C.c_float,
                                 C.c_float, C.c_int, _P, C.c_int]),
+    "or_bh_wave_ranges": (_I, [_I, _I, _P, _P, _P, _P, C.c_int, _P, _P]),
     "or_bh_pipeline_group": (_I, [_I, _P, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
                                   _P, _P, _P, _P, C.c_int]),
     "or_bh_pipeline": (_I, [_I, _P, C.c_float, C.c_float, C.c_float, C.c_int, _P, _P, _P, _P, C.c_int]),
@@ -222,6 +223,15 @@ def bh_pipeline(pos4, eps2, theta=0.5, leaf_size=8, G=1.0, nthreads=None, group=
                                         1 if cubic else 0, _p(out), _p(codes), _p(perm), C.byref(cnt),
                                         nthreads or threads())
     return out, codes, perm, cnt.value, int(inter)
+
+
+def bh_wave_ranges(tree, n, cap=64):
+    k = tree["start"].shape[0]
+    rs = np.empty(max(n, 1), np.int32)
+    rc = np.empty(max(n, 1), np.int32)
+    r = load().or_bh_wave_ranges(n, k, _p(tree["first_child"]), _p(tree["start"]), _p(tree["count"]),
+                                 _p(tree["level"]), cap, _p(rs), _p(rc))
+    return rs[:r].copy(), rc[:r].copy()
 
 
 def bh_forces_group(pos4s, perm, bmin, bmax, tree, com4, eps2, theta=0.5, G=1.0, group=64,

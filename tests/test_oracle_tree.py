@@ -290,3 +290,27 @@ def test_cubic_box_is_the_reference_formula_on_a_cube(ora):
     f0, *_, inter0 = ora.bh_pipeline(pos, 1.0, theta=1e-7, leaf_size=8, group=64, cubic=True)
     ref_f = ora.direct_f64(pos[:, :3].astype(np.float64), np.ones(2000), 1.0)
     assert inter0 == 2000 * 1999 and rel_err(f0, ref_f).max() < 1e-5
+
+
+def test_wave_ranges_partition_the_bodies(ora):
+    rng = np.random.default_rng(24)
+    pos = vec4(seeded_ball(5000, seed=25))
+    pos[100:300, :3] = pos[100, :3]  # 200 coincident bodies: one unsplittable leaf
+    mn, mx = ora.bbox(pos)
+    perm, srt = ora.sort_codes(ora.morton(pos, mn, mx, cubic=True))
+    t = ora.build_tree(srt, 8)
+    rs, rc = ora.bh_wave_ranges(t, 5000, 64)
+    assert rc.min() >= 1 and rc.max() <= 64
+    cover = np.zeros(5000, int)
+    for a, c in zip(rs, rc):
+        cover[a:a + c] += 1
+    assert np.all(cover == 1)
+    # each range lies in one cell: its bodies share a code prefix at the parent's level
+    for a, c in zip(rs, rc):
+        x = srt[a] ^ srt[a + c - 1]
+        assert x < (1 << 30)
+    # a small system is one range; capacity 1 gives singletons
+    rs1, rc1 = ora.bh_wave_ranges(ora.build_tree(srt[:50], 8), 50, 64)
+    assert list(rs1) == [0] and list(rc1) == [50]
+    rs2, rc2 = ora.bh_wave_ranges(t, 5000, 1)
+    assert len(rs2) == 5000 and np.all(rc2 == 1)
