@@ -99,7 +99,41 @@ def cpu_baseline(pos, mass, budget_s: float):
     }
 
 
-def bench_bh(nb, steps: int, warmup: int):
+def cpu_baseline_bh(pos, mass, budget_s: float):
+    """v6 (ParallelOctreeCPU.cpp:7-505, ParticleSolverBHutCPUParallel.cpp:18-41)
+    Barnes-Hut force pass on this host's cores: the whole tree build, then the
+    force loop over a bounded set of targets, extrapolated to one full step."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as ora
+
+    n = pos.shape[0]
+    p4 = np.zeros((n, 4), np.float32)
+    p4[:, :3] = pos
+    m4 = np.zeros((n, 4), np.float32)
+    m4[:, 0] = mass
+    th = host_threads()
+    cal = 256 * th
+    rc, _, build_s, force_s = ora.v6_forces(p4, m4, EPS2, 0.5, t0=0, t1=cal, nthreads=th, fast=True)
+    if rc != 0:
+        return None
+    targets = int(min(n, max(cal, cal * budget_s / max(force_s, 1e-9))))
+    rc, _, build_s, force_s = ora.v6_forces(p4, m4, EPS2, 0.5, t0=0, t1=targets, nthreads=th,
+                                            fast=True)
+    if rc != 0:
+        return None
+    step_s = build_s + force_s * n / targets
+    return {
+        "value": 1e3 * step_s,
+        "unit": "ms/step",
+        "cores": th,
+        "kind": "port",
+        "sample": f"v6 restatement (-O3 -ffast-math, OpenMP): full tree build {build_s:.2f} s, force "
+                  f"loop over targets [0, {targets}) of the same discs in {force_s:.1f} s, "
+                  f"extrapolated to {n} targets",
+    }
+
+
+def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0):
     n = 1 << 22
     t0 = time.perf_counter()
     pos, vel, m = nb.ic_discs(2024, n)
@@ -116,6 +150,10 @@ def bench_bh(nb, steps: int, warmup: int):
         wall = time.perf_counter() - t0
         t = sim.timing()
     ms = 1e3 * wall / steps
+    cpu = None
+    if cpu_budget > 0:
+        log("bh: cpu baseline (oracle v6 restatement)")
+        cpu = cpu_baseline_bh(pos.astype(np.float32), m, cpu_budget)
     return {
         "workload": "Barnes-Hut N=4194304, two exponential discs (Rd=30, h=2, 300 apart, "
                     "closing 0.5/step), theta=0.5, eps2=40, dt=0.0035",
@@ -126,6 +164,7 @@ def bench_bh(nb, steps: int, warmup: int):
         "traverse_ms": t["force_ms"] / max(1, t["force_calls"]),
         "interactions_first_step": inter,
         "interactions_per_s": inter / (t["force_ms"] / max(1, t["force_calls"]) / 1e3),
+        "cpu_baseline": cpu,
     }
 
 
@@ -194,12 +233,19 @@ def poisson_jobs(seed: int, rate: float, duration: float):
     return jobs
 
 
-def run_stream(nb, jobs, limit_s: float, one_at_a_time: bool):
-    """Feed the job stream in real time; stop admitting work after limit_s.
-    Returns per-job records and totals."""
-    kw = dict(max_jobs=1) if one_at_a_time else {}
+def job_ic(nb, j):
+    """The job's seeded initial conditions, as the server makes them."""
+    if j["ic"] == nb.IC_PLUMMER:
+        return nb.ic_plummer(j["seed"], j["n"], 50.0, 1e4)
+    return nb.ic_ball(j["seed"], j["n"], 100.0, 1.0)
+
+
+def run_stream(nb, jobs, limit_s: float, **server_kw):
+    """Batched mode: feed the job stream to nb_server in real time for limit_s
+    seconds. Returns finished-job records, body-steps done (finished jobs and the
+    steps of unfinished ones), and the submitted / queued / running counts."""
     done, body_steps = [], 0
-    with nb.Server(**kw) as s:
+    with nb.Server(**server_kw) as s:
         ids, k = {}, 0
         t_start = time.perf_counter()
         while True:
@@ -216,27 +262,88 @@ def run_stream(nb, jobs, limit_s: float, one_at_a_time: bool):
                     st = s.status(jid)
                     if st["state"] == nb.JOB_DONE:
                         done.append({"n": ids[jid]["n"], "steps": st["steps"],
-                                     "latency": st["t_done"] - st["t_submit"],
-                                     "solver": st["solver"]})
+                                     "latency": st["t_done"] - st["t_submit"]})
                         body_steps += ids[jid]["n"] * st["steps"]
                         s.forget(jid)
                         del ids[jid]
             elif running == 0 and k < len(jobs):
                 time.sleep(min(0.01, max(0.0, jobs[k]["t"] - now)))
-        queued = sum(1 for jid in ids if s.status(jid)["state"] == nb.JOB_QUEUED)
-        running = sum(1 for jid in ids if s.status(jid)["state"] == nb.JOB_RUNNING)
+        queued = running = 0
+        for jid, j in ids.items():
+            st = s.status(jid)
+            queued += st["state"] == nb.JOB_QUEUED
+            running += st["state"] == nb.JOB_RUNNING
+            body_steps += j["n"] * st["steps_done"]
     return done, body_steps, k, queued, running
 
 
-def bench_serve(nb, rate: float, duration: float, limit: float):
+def run_one_at_a_time(nb, jobs, limit_s: float, ic_threads: int):
+    """The baseline the server is compared with: the same stream run one job at
+    a time, each job alone on the device start to finish (nb_create, upload,
+    its steps replayed in chunks of 32, FP64 download, nb_destroy), with the
+    initial conditions of arrived jobs prepared ahead on ic_threads threads."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    done, body_steps = [], 0
+    futs, k, k_prep = {}, 0, 0
+    with ThreadPoolExecutor(ic_threads) as ex:
+        t_start = time.perf_counter()
+
+        def feed():
+            now = time.perf_counter() - t_start
+            nonlocal k_prep
+            while k_prep < len(jobs) and k_prep < k + 2 * ic_threads and jobs[k_prep]["t"] <= now:
+                futs[k_prep] = ex.submit(job_ic, nb, jobs[k_prep])
+                k_prep += 1
+            return now
+
+        while k < len(jobs):
+            now = feed()
+            if now > limit_s:
+                break
+            if k not in futs:
+                time.sleep(min(0.01, max(0.0, jobs[k]["t"] - now)))
+                continue
+            j = jobs[k]
+            pos, vel, m = futs.pop(k).result()
+            solver = nb.SOLVER_BH if j["n"] >= 65536 else nb.SOLVER_DIRECT
+            left = j["steps"]
+            with nb.Simulation(j["n"], EPS2, DT, solver=solver) as sim:
+                sim.upload(pos, vel, m)
+                while left > 0 and time.perf_counter() - t_start <= limit_s:
+                    c = min(32, left)
+                    sim.step(c)
+                    sim.sync()
+                    left -= c
+                    body_steps += j["n"] * c
+                    feed()
+                if left == 0:
+                    sim.download_f64()
+                    done.append({"n": j["n"], "steps": j["steps"],
+                                 "latency": time.perf_counter() - t_start - j["t"]})
+            k += 1
+        for f in futs.values():
+            f.cancel()
+    queued = sum(1 for j in jobs[k:] if j["t"] <= limit_s)
+    return done, body_steps, min(len(jobs), k_prep), queued, int(k < len(jobs))
+
+
+def bench_serve(nb, rate: float, duration: float, limit: float, lookaheads=(16, 4096)):
     jobs = poisson_jobs(5050, rate, duration)
     out = {"workload": f"Poisson stream {rate:g} jobs/s for {duration:g} s (seeded), N log-uniform "
                        f"[1024, 2^20], direct below 65536 else Barnes-Hut theta=0.5, steps U[10,500], "
-                       f"Plummer or ball, eps2=40, dt=0.0035; both modes measured for {limit:g} s",
-           "offered_jobs": len(jobs)}
-    for mode, one in (("batched", False), ("one_at_a_time", True)):
+                       f"Plummer or ball, eps2=40, dt=0.0035; every mode measured for {limit:g} s",
+           "offered_jobs": len(jobs),
+           "offered_body_steps": int(sum(j["n"] * j["steps"] for j in jobs if j["t"] <= limit))}
+    ic_threads = 8
+    modes = [("one_at_a_time", None)] + [(f"batched_lookahead{la}", la) for la in lookaheads]
+    for mode, la in modes:
         log(f"serve: {mode}")
-        done, bs, submitted, queued, running = run_stream(nb, jobs, limit, one)
+        if la is None:
+            done, bs, submitted, queued, running = run_one_at_a_time(nb, jobs, limit, ic_threads)
+        else:
+            done, bs, submitted, queued, running = run_stream(nb, jobs, limit, lookahead=la,
+                                                              ic_threads=ic_threads)
         lat = np.array([d["latency"] for d in done]) if done else np.array([np.nan])
         out[mode] = {"submitted": submitted, "completed": len(done), "running": running,
                      "queued": queued, "body_steps_per_s": bs / limit,
@@ -244,9 +351,11 @@ def bench_serve(nb, rate: float, duration: float, limit: float):
                      "latency_p50_s": float(np.percentile(lat, 50)),
                      "latency_p99_s": float(np.percentile(lat, 99))}
         log(f"serve: {mode}: {out[mode]}")
-    b, o = out["batched"], out["one_at_a_time"]
-    out["throughput_ratio_body_steps"] = b["body_steps_per_s"] / max(o["body_steps_per_s"], 1e-9)
-    out["throughput_ratio_jobs"] = b["completed_per_s"] / max(o["completed_per_s"], 1e-9)
+    o = out["one_at_a_time"]
+    for mode, la in modes[1:]:
+        b = out[mode]
+        b["ratio_body_steps"] = b["body_steps_per_s"] / max(o["body_steps_per_s"], 1e-9)
+        b["ratio_jobs"] = b["completed_per_s"] / max(o["completed_per_s"], 1e-9)
     return out
 
 
@@ -374,7 +483,7 @@ def main():
     }
     if not args.no_bh:
         log("bh: N=2^22 two-disc collision")
-        result["bh_4m"] = bench_bh(nb, args.bh_steps, 2)
+        result["bh_4m"] = bench_bh(nb, args.bh_steps, 2, 0.0 if args.no_cpu else args.cpu_budget)
         log(f"bh: {result['bh_4m']['ms_per_step']:.2f} ms/step")
     if not args.no_batch:
         log("batch: C4 ensemble")

@@ -206,9 +206,12 @@ int nb_batch_get_timing(nb_batch* b, nb_timing* out);
 
 /* ---- serving ---------------------------------------------------------------
  * A request loop over many simulation jobs: jobs are submitted at any time,
- * admitted in arrival order while their device memory fits the HBM budget (a
- * bounded look-ahead lets small jobs pass a large one that does not fit yet),
- * and every admitted job advances one step per nb_server_tick. Direct-sum jobs
+ * admitted in arrival order while their device memory fits the HBM budget and
+ * their estimated step time fits the tick budget (a bounded look-ahead lets
+ * small jobs pass a large one that does not fit yet), and every admitted job
+ * advances one step per nb_server_tick. The tick budget keeps a tick short, so
+ * admitted jobs finish in about steps x tick_budget_ms instead of sharing the
+ * device with every queued job at once. Direct-sum jobs
  * (n < bh_threshold) share one nb_batch launch; Barnes-Hut jobs run their own
  * contexts on their own streams, so their kernels overlap on the device.
  * Finished jobs release their memory and keep their FP64 final state on the
@@ -223,6 +226,9 @@ typedef struct nb_server_config {
   int32_t leaf_size;     /* Barnes-Hut bodies per leaf; 0 = default */
   int32_t ic_threads;    /* host threads preparing initial conditions; 0 = 4 */
   int32_t lookahead;     /* queued jobs considered per admission pass; 0 = 16 */
+  float tick_budget_ms;  /* estimated device time of one tick's steps; 0 = 10 ms.
+                            A job larger than the budget runs when it is alone. */
+  int32_t pad_;
 } nb_server_config;
 
 #define NB_IC_PLUMMER 0 /* a = 50, total mass 1e4, truncated at 10 a */

@@ -1,20 +1,26 @@
 // nb_serve.hip — the serving loop of nbody.h (nb_server_*): continuous
 // batching of simulation jobs on one GPU.
 //
-// Each tick:
-//   1. admit   scan the first `lookahead` queued jobs in arrival order; a job
-//              whose initial conditions are ready and whose device memory fits
-//              the remaining HBM budget is admitted: direct-sum jobs become
-//              clusters of the shared nb_batch pool, Barnes-Hut jobs get their own
-//              nb_ctx (own stream);
-//   2. step    one nb_batch_step for every direct-sum job at once (one persistent
+// A job moves queued -> preparing -> running -> done. Each tick:
+//   1. select  scan the first `lookahead` queued jobs in arrival order; a job
+//              whose device memory fits what is left of the HBM budget (Barnes-Hut
+//              contexts) or of the shared direct-sum pool, and whose estimated
+//              step time fits what is left of the tick budget, is selected: its
+//              memory and tick time are reserved and a host worker generates its
+//              initial conditions. A job estimated above the whole tick budget is
+//              selected only onto an idle device. Later, smaller jobs may pass a
+//              job that does not fit yet (bounded by the look-ahead);
+//   2. start   selected jobs whose initial conditions are ready are uploaded:
+//              direct-sum jobs become clusters of the shared nb_batch pool,
+//              Barnes-Hut jobs get their own nb_ctx (own stream);
+//   3. step    one nb_batch_step for every direct-sum job at once (one persistent
 //              force launch) and one nb_step per Barnes-Hut job, all enqueued
 //              before any wait, so Barnes-Hut kernels overlap with each other and
 //              with the batch;
-//   3. retire  jobs that reached their step count copy their FP64 state to the
-//              host and release their device memory and budget.
-// Initial conditions are generated on host worker threads ahead of admission
-// (at most 2 * lookahead jobs prepared and waiting), in arrival order.
+//   4. retire  jobs that reached their step count copy their FP64 state to the
+//              host and release their reservations.
+// Only selected jobs hold host memory for initial conditions, so a long queue
+// costs a few bytes per job.
 #include <algorithm>
 #include <atomic>
 #include <chrono>
@@ -34,15 +40,18 @@ namespace {
 
 struct Job {
   nb_job desc{};
-  int32_t state = NB_JOB_QUEUED;
+  int32_t state = NB_JOB_QUEUED;  // preparing jobs report NB_JOB_QUEUED
   int32_t solver = NB_SOLVER_DIRECT;
   int32_t steps_done = 0;
   double t_submit = 0, t_admit = 0, t_done = 0;
-  int64_t bytes = 0;  // budget charged (Barnes-Hut contexts)
-  int32_t slot = -1;  // batch cluster id
+  // reservations held from selection to retirement
+  int64_t bytes = 0;        // HBM budget (Barnes-Hut contexts)
+  int64_t pool_bodies = 0;  // direct-sum pool bodies (padded)
+  double cost_ms = 0;       // estimated device time of one step
+  int32_t slot = -1;        // batch cluster id
   nb_ctx* ctx = nullptr;
-  // initial conditions (prepared by a worker) and, later, the final state
-  std::atomic<int> prepared{0};  // 0 waiting, 1 ready, -1 failed
+  // initial conditions (made by a worker) and, later, the final state
+  std::atomic<int> prepared{0};  // 0 pending, 1 ready, -1 failed
   std::vector<double> pos, vel;
   std::vector<float> mass;
 };
@@ -62,23 +71,37 @@ int64_t bh_bytes(int64_t n) {
   return np * (16 + 4) + n * (4 * 24 + 4 * 5 + 16 + 32 + 8) + 32 * tree_bound(n) + (4 << 20);
 }
 
+int64_t padded(int64_t n) { return (n + kPadBodies - 1) / kPadBodies * kPadBodies; }
+
+// Estimated device time of one step, from rates measured on MI355X: the
+// compensated direct sum runs about 3.2e12 pair interactions/s once the device
+// is full, and a Barnes-Hut step (theta = 0.5, build + traversal) costs about
+// 2 ns per body plus some 60 us of launches (8.5 ms at 4,194,304 bodies).
+double step_cost_ms(int64_t n, int32_t solver) {
+  if (solver == NB_SOLVER_DIRECT) return 2e-3 + (double)n * (double)n / 3.2e9;
+  return 0.06 + (double)n * 2.0e-6;
+}
+
 }  // namespace
 
 struct nb_server {
   nb_server_config cfg{};
   std::chrono::steady_clock::time_point t0;
   nb_batch* batch = nullptr;
-  int64_t budget_left = 0;
+  int64_t budget_total = 0;  // HBM budget beyond the direct-sum pool
+  int64_t bytes_left = 0;    // of budget_total, not reserved
+  int64_t pool_left = 0;     // direct-sum pool bodies not reserved
+  double load_ms = 0;        // reserved tick time (preparing and running jobs)
   int64_t next_id = 0;
   std::map<int64_t, std::unique_ptr<Job>> jobs;
-  std::deque<int64_t> queue;      // queued, arrival order
-  std::vector<int64_t> running;   // admitted
+  std::deque<int64_t> queue;       // not yet selected, arrival order
+  std::vector<int64_t> preparing;  // selected, initial conditions pending or ready
+  std::vector<int64_t> running;    // on the device
   // initial-condition workers
   std::mutex mu;
   std::condition_variable cv;        // workers wait for jobs to prepare
-  std::condition_variable ready_cv;  // tick waits for the head job when idle
-  std::deque<int64_t> prep_queue;
-  int prepared_waiting = 0;
+  std::condition_variable ready_cv;  // an idle tick waits for a prepared job
+  std::deque<Job*> prep_queue;
   bool stop = false;
   std::vector<std::thread> workers;
 
@@ -91,23 +114,26 @@ struct nb_server {
       Job* j = nullptr;
       {
         std::unique_lock<std::mutex> lk(mu);
-        cv.wait(lk, [&] {
-          return stop || (!prep_queue.empty() && prepared_waiting < 2 * cfg.lookahead);
-        });
+        cv.wait(lk, [&] { return stop || !prep_queue.empty(); });
         if (stop) return;
-        j = jobs.at(prep_queue.front()).get();
+        j = prep_queue.front();
         prep_queue.pop_front();
-        ++prepared_waiting;
       }
       const int64_t n = j->desc.n;
-      j->pos.resize(3 * n);
-      j->vel.resize(3 * n);
-      j->mass.resize(n);
-      const int rc =
-          j->desc.ic == NB_IC_BALL
-              ? nb_ic_ball(j->desc.seed, n, 100.0, 1.0, j->pos.data(), j->vel.data(), j->mass.data())
-              : nb_ic_plummer(j->desc.seed, n, 50.0, 1e4, 10.0, j->desc.G == 0.f ? 1.0 : j->desc.G,
-                              j->pos.data(), j->vel.data(), j->mass.data());
+      int rc = -1;
+      try {
+        j->pos.resize(3 * n);
+        j->vel.resize(3 * n);
+        j->mass.resize(n);
+        rc = j->desc.ic == NB_IC_BALL
+                 ? nb_ic_ball(j->desc.seed, n, 100.0, 1.0, j->pos.data(), j->vel.data(),
+                              j->mass.data())
+                 : nb_ic_plummer(j->desc.seed, n, 50.0, 1e4, 10.0,
+                                 j->desc.G == 0.f ? 1.0 : j->desc.G, j->pos.data(),
+                                 j->vel.data(), j->mass.data());
+      } catch (const std::bad_alloc&) {
+        rc = -1;
+      }
       {
         std::lock_guard<std::mutex> lk(mu);
         j->prepared.store(rc == 0 ? 1 : -1);
@@ -115,11 +141,22 @@ struct nb_server {
       ready_cv.notify_all();
     }
   }
+
+  void release(Job& j) {
+    bytes_left += j.bytes;
+    pool_left += j.pool_bodies;
+    load_ms -= j.cost_ms;
+    if (load_ms < 1e-9) load_ms = 0;
+    j.bytes = j.pool_bodies = 0;
+    j.cost_ms = 0;
+  }
 };
 
 namespace {
 
-int admit(nb_server* s, Job& j) {
+// Upload a prepared job. NB_E_BUDGET: the pool has no free segment of that
+// size yet (fragmentation); the job stays prepared and is retried next tick.
+int start(nb_server* s, Job& j) {
   const int64_t n = j.desc.n;
   if (j.solver == NB_SOLVER_DIRECT) {
     int32_t id = -1;
@@ -128,8 +165,6 @@ int admit(nb_server* s, Job& j) {
     if (rc != NB_OK) return rc;
     j.slot = id;
   } else {
-    const int64_t need = bh_bytes(n);
-    if (need > s->budget_left) return NB_E_BUDGET;
     nb_config c{};
     c.n = n;
     c.eps2 = j.desc.eps2;
@@ -146,8 +181,6 @@ int admit(nb_server* s, Job& j) {
       j.ctx = nullptr;
       return rc;
     }
-    j.bytes = need;
-    s->budget_left -= need;
   }
   // the host copy of the initial conditions is no longer needed
   std::vector<double>().swap(j.pos);
@@ -157,22 +190,30 @@ int admit(nb_server* s, Job& j) {
 
 int retire(nb_server* s, Job& j) {
   const int64_t n = j.desc.n;
-  j.pos.resize(3 * n);
-  j.vel.resize(3 * n);
-  int rc;
+  int rc = NB_OK;
   if (j.solver == NB_SOLVER_DIRECT) {
-    rc = nb_batch_download(s->batch, j.slot, j.pos.data(), j.vel.data(), nullptr);
-    if (rc == NB_OK) rc = nb_batch_remove(s->batch, j.slot);
-    j.slot = -1;
-  } else {
+    if (j.slot >= 0) {
+      j.pos.resize(3 * n);
+      j.vel.resize(3 * n);
+      rc = nb_batch_download(s->batch, j.slot, j.pos.data(), j.vel.data(), nullptr);
+      const int rr = nb_batch_remove(s->batch, j.slot);
+      if (rc == NB_OK) rc = rr;
+      j.slot = -1;
+    }
+  } else if (j.ctx) {
+    j.pos.resize(3 * n);
+    j.vel.resize(3 * n);
     rc = nb_download_f64(j.ctx, j.pos.data(), j.vel.data(), nullptr);
     nb_destroy(j.ctx);
     j.ctx = nullptr;
-    s->budget_left += j.bytes;
-    j.bytes = 0;
   }
   std::vector<float>().swap(j.mass);
+  s->release(j);
   if (j.state != NB_JOB_FAILED) j.state = rc == NB_OK ? NB_JOB_DONE : NB_JOB_FAILED;
+  if (j.state == NB_JOB_FAILED) {
+    std::vector<double>().swap(j.pos);
+    std::vector<double>().swap(j.vel);
+  }
   j.t_done = s->now();
   return rc;
 }
@@ -208,6 +249,7 @@ int nb_server_create(nb_server** out, const nb_server_config* cfg) {
   if (s->cfg.theta <= 0.f) s->cfg.theta = 0.5f;
   if (s->cfg.ic_threads <= 0) s->cfg.ic_threads = 4;
   if (s->cfg.lookahead <= 0) s->cfg.lookahead = 16;
+  if (!(s->cfg.tick_budget_ms > 0.f)) s->cfg.tick_budget_ms = 10.f;
   s->t0 = std::chrono::steady_clock::now();
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
@@ -233,7 +275,8 @@ int nb_server_create(nb_server** out, const nb_server_config* cfg) {
     delete s;
     return rc;
   }
-  s->budget_left = budget - pool_bytes;
+  s->budget_total = s->bytes_left = budget - pool_bytes;
+  s->pool_left = s->cfg.direct_pool;
   for (int t = 0; t < s->cfg.ic_threads; ++t) s->workers.emplace_back([s] { s->worker(); });
   *out = s;
   return NB_OK;
@@ -248,18 +291,21 @@ int nb_server_submit(nb_server* s, const nb_job* job, int64_t* id) {
   if (job->n < 1 || job->n >= (int64_t(1) << 27) || job->steps < 0 ||
       (job->ic != NB_IC_PLUMMER && job->ic != NB_IC_BALL) || !(job->eps2 >= 0.f))
     return nb_fail(NB_E_ARG, "nb_server_submit: bad job description");
+  const int32_t solver = job->n >= s->cfg.bh_threshold ? NB_SOLVER_BH : NB_SOLVER_DIRECT;
+  if (solver == NB_SOLVER_BH ? bh_bytes(job->n) > s->budget_total
+                             : padded(job->n) > s->cfg.direct_pool)
+    return nb_fail(NB_E_BUDGET, "nb_server_submit: the job can never fit the HBM budget");
   auto j = std::make_unique<Job>();
   j->desc = *job;
-  j->solver = job->n >= s->cfg.bh_threshold ? NB_SOLVER_BH : NB_SOLVER_DIRECT;
+  j->solver = solver;
+  j->cost_ms = step_cost_ms(job->n, solver);
   j->t_submit = s->now();
   const int64_t jid = s->next_id++;
   {
     std::lock_guard<std::mutex> lk(s->mu);
     s->jobs[jid] = std::move(j);
-    s->prep_queue.push_back(jid);
   }
   s->queue.push_back(jid);
-  s->cv.notify_one();
   *id = jid;
   return NB_OK;
 }
@@ -267,54 +313,71 @@ int nb_server_submit(nb_server* s, const nb_job* job, int64_t* id) {
 int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
   if (!s) return nb_fail(NB_E_ARG, "nb_server_tick: null server");
   int finished = 0;
-  // Idle with queued work: wait (up to 50 ms) for the head job's initial
-  // conditions instead of returning an empty tick.
-  if (s->running.empty() && !s->queue.empty()) {
-    Job& head = *s->jobs.at(s->queue.front());
-    std::unique_lock<std::mutex> lk(s->mu);
-    s->ready_cv.wait_for(lk, std::chrono::milliseconds(50), [&] { return head.prepared.load() != 0; });
-  }
-  // 1. admit (arrival order, bounded look-ahead)
+  // 1. select (arrival order, bounded look-ahead)
   int scanned = 0;
-  for (auto it = s->queue.begin();
-       it != s->queue.end() && scanned < s->cfg.lookahead &&
-       (int)s->running.size() < s->cfg.max_jobs;) {
+  for (auto it = s->queue.begin(); it != s->queue.end() && scanned < s->cfg.lookahead;) {
+    if ((int)(s->running.size() + s->preparing.size()) >= s->cfg.max_jobs) break;
     ++scanned;
     Job& j = *s->jobs.at(*it);
-    const int p = j.prepared.load();
-    if (p == 0) break;  // preparation is in arrival order: later ones are not ready either
-    if (p < 0) {
-      j.state = NB_JOB_FAILED;
-      j.t_done = s->now();
-      {
-        std::lock_guard<std::mutex> lk(s->mu);
-        --s->prepared_waiting;
-      }
-      s->cv.notify_one();
-      it = s->queue.erase(it);
+    const bool idle = s->running.empty() && s->preparing.empty();
+    const bool fits_tick = idle || s->load_ms + j.cost_ms <= s->cfg.tick_budget_ms;
+    const bool fits_mem = j.solver == NB_SOLVER_BH ? bh_bytes(j.desc.n) <= s->bytes_left
+                                                   : padded(j.desc.n) <= s->pool_left;
+    if (!fits_tick || !fits_mem) {
+      ++it;  // a later, smaller job may fit
       continue;
     }
-    const int rc = admit(s, j);
+    if (j.solver == NB_SOLVER_BH) {
+      j.bytes = bh_bytes(j.desc.n);
+      s->bytes_left -= j.bytes;
+    } else {
+      j.pool_bodies = padded(j.desc.n);
+      s->pool_left -= j.pool_bodies;
+    }
+    s->load_ms += j.cost_ms;
+    s->preparing.push_back(*it);
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      s->prep_queue.push_back(&j);
+    }
+    s->cv.notify_one();
+    it = s->queue.erase(it);
+  }
+  // Idle with work being prepared: wait (up to 50 ms) for the first prepared
+  // job instead of returning an empty tick.
+  if (s->running.empty() && !s->preparing.empty()) {
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->ready_cv.wait_for(lk, std::chrono::milliseconds(50), [&] {
+      for (int64_t id : s->preparing)
+        if (s->jobs.at(id)->prepared.load() != 0) return true;
+      return false;
+    });
+  }
+  // 2. start the prepared jobs (selection order)
+  for (auto it = s->preparing.begin(); it != s->preparing.end();) {
+    Job& j = *s->jobs.at(*it);
+    const int p = j.prepared.load();
+    if (p == 0) {
+      ++it;
+      continue;
+    }
+    const int rc = p < 0 ? NB_E_OOM : start(s, j);
     if (rc == NB_E_BUDGET) {
       ++it;
       continue;
     }
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      --s->prepared_waiting;
-    }
-    s->cv.notify_one();
     if (rc != NB_OK) {
       j.state = NB_JOB_FAILED;
-      j.t_done = s->now();
+      retire(s, j);
+      ++finished;
     } else {
       j.state = NB_JOB_RUNNING;
       j.t_admit = s->now();
       s->running.push_back(*it);
     }
-    it = s->queue.erase(it);
+    it = s->preparing.erase(it);
   }
-  // 2. one step for every running job, everything enqueued before any wait
+  // 3. one step for every running job, everything enqueued before any wait
   bool any_direct = false;
   for (int64_t id : s->running) {
     Job& j = *s->jobs.at(id);
@@ -346,7 +409,7 @@ int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
     const int rc = nb_batch_sync(s->batch);
     if (rc != NB_OK) return rc;
   }
-  // 3. retire
+  // 4. retire
   for (auto it = s->running.begin(); it != s->running.end();) {
     Job& j = *s->jobs.at(*it);
     if (j.steps_done >= j.desc.steps || j.state == NB_JOB_FAILED) {

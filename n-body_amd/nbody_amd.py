@@ -121,6 +121,8 @@ class ServerConfig(C.Structure):
         ("leaf_size", C.c_int32),
         ("ic_threads", C.c_int32),
         ("lookahead", C.c_int32),
+        ("tick_budget_ms", C.c_float),
+        ("pad_", C.c_int32),
     ]
 
 
@@ -578,11 +580,12 @@ class Server:
 
     def __init__(self, hbm_budget: int = 0, direct_pool: int = 0, max_jobs: int = 0,
                  bh_threshold: int = 0, theta: float = 0.0, leaf_size: int = 0, ic_threads: int = 0,
-                 lookahead: int = 0):
+                 lookahead: int = 0, tick_budget_ms: float = 0.0):
         self._lib = load()
         cfg = ServerConfig(hbm_budget=hbm_budget, direct_pool=direct_pool, max_jobs=max_jobs,
                            bh_threshold=bh_threshold, theta=theta, leaf_size=leaf_size,
-                           ic_threads=ic_threads, lookahead=lookahead)
+                           ic_threads=ic_threads, lookahead=lookahead,
+                           tick_budget_ms=tick_budget_ms)
         self._s = C.c_void_p()
         _check(self._lib.nb_server_create(C.byref(self._s), C.byref(cfg)))
         self.sizes: dict[int, int] = {}

@@ -77,3 +77,30 @@ def test_budget_limits_concurrency(nb):
                 break
         assert all(s.status(i)["state"] == nb.JOB_DONE for i in ids)
     assert max_running == 1
+
+
+def test_tick_budget_limits_concurrency(nb):
+    # direct-sum jobs of 20000 bodies are estimated at 0.127 ms a step each: a
+    # 0.2 ms tick budget runs them one after another (no two run intervals
+    # overlap), a 1 ms budget selects all three at once (their 300-step run
+    # intervals overlap)
+    jobs = [(20000, 300, 1, 21), (20000, 300, 1, 22), (20000, 300, 1, 23)]
+    for budget, serial in ((0.2, True), (1.0, False)):
+        with nb.Server(direct_pool=1 << 17, tick_budget_ms=budget) as s:
+            ids = [s.submit(*j) for j in jobs]
+            t0 = time.monotonic()
+            while time.monotonic() - t0 < 60:
+                s.tick()
+                if all(s.status(i)["state"] == nb.JOB_DONE for i in ids):
+                    break
+            st = [s.status(i) for i in ids]
+        assert all(x["state"] == nb.JOB_DONE for x in st)
+        spans = sorted((x["t_admit"], x["t_done"]) for x in st)
+        overlaps = [spans[k + 1][0] < spans[k][1] for k in range(2)]
+        assert overlaps == [not serial] * 2, (budget, spans)
+
+
+def test_job_that_never_fits_is_refused(nb):
+    with nb.Server(direct_pool=1 << 12, bh_threshold=8192) as s:
+        with pytest.raises(nb.NBodyError):
+            s.submit(5000, 2, 0, 1)  # 5120 padded bodies > the 4096-body pool
