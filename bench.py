@@ -43,6 +43,17 @@ def log(msg: str):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def host_cpu() -> str:
+    """CPU model of this host, for the baseline records."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def host_threads() -> int:
     env = os.environ.get("OMP_NUM_THREADS")
     if env and env.isdigit():
@@ -94,6 +105,7 @@ def cpu_baseline(pos, mass, budget_s: float):
         "unit": "interactions/s",
         "cores": th,
         "kind": "port",
+        "cpu": host_cpu(),
         "sample": f"v2 force pass (OpenMP static, -O3 -ffast-math) over {targets} targets x {n} "
                   f"sources of the same Plummer input, {el:.1f} s",
     }
@@ -127,6 +139,7 @@ def cpu_baseline_bh(pos, mass, budget_s: float):
         "unit": "ms/step",
         "cores": th,
         "kind": "port",
+        "cpu": host_cpu(),
         "sample": f"v6 restatement (-O3 -ffast-math, OpenMP): full tree build {build_s:.2f} s, force "
                   f"loop over targets [0, {targets}) of the same discs in {force_s:.1f} s, "
                   f"extrapolated to {n} targets",

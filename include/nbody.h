@@ -126,9 +126,29 @@ int nb_energy(nb_ctx* ctx, double* ekin, double* epot, double p[3]);
 int nb_save_snapshot(nb_ctx* ctx, const char* path);
 int nb_load_snapshot(nb_ctx* ctx, const char* path);
 
+/* Morton reordering of the resident state (replaces ParticleSystem::gpuSort,
+ * ParticleSystem.cpp:106-171, which is compiled but never called there): the
+ * device slots of positions, velocities, accelerations, forces and masses are
+ * permuted into the stable (Morton code, slot) order of the current positions,
+ * so that bodies close in space are close in memory. Either solver. Uploads,
+ * downloads, forces and snapshots keep the caller's body order; only the debug
+ * taps below and nb_get_order see slots. An upload returns the slots to the
+ * caller's order. */
+int nb_reorder(nb_ctx* ctx);
+/* order[slot] = the caller's index of the body in that slot (identity until
+ * nb_reorder). */
+int nb_get_order(nb_ctx* ctx, int32_t* order);
+
+/* Headless render consumer: float[4*n] = (x, y, z, |v|) per body in the
+ * caller's order, FP32, from the FP64 state. These are the inputs of the
+ * reference's point renderer (vertexShader.glsl:3-9 reads positions and
+ * velocities at bindings 0/1; fragmentShader.glsl colours by length(velocity)),
+ * which this engine leaves to the consumer. */
+int nb_render_view(nb_ctx* ctx, float* xyzs);
+
 /* ---- Barnes-Hut debug taps (valid after a BH nb_step or nb_forces) ---- */
-/* codes[n]: 30-bit Morton code of body i (unsorted order);
- * perm[n]: body index at sorted position k (sorted by (code, index)). */
+/* codes[n]: 30-bit Morton code of the body in slot i (unsorted order);
+ * perm[n]: slot at sorted position k (sorted by (code, slot)). */
 int nb_debug_morton(nb_ctx* ctx, uint32_t* codes, uint32_t* perm);
 /* Flattened octree, breadth-first numbering, children in Morton-digit order. */
 typedef struct nb_node_view {

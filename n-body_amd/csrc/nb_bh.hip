@@ -454,6 +454,31 @@ __global__ __launch_bounds__(256) void k_gather(const float4* __restrict__ pm,
   }
 }
 
+// Morton reordering of the resident state: dst[k] = src[perm[k]] per component.
+__global__ __launch_bounds__(256) void k_permute_d3(const double* __restrict__ src,
+                                                    const uint32_t* __restrict__ perm, int n,
+                                                    double* __restrict__ dst) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    const uint32_t p = perm[i];
+    if (p < (uint32_t)n) {
+      dst[i] = src[p];
+      dst[n + i] = src[n + p];
+      dst[2 * n + i] = src[2 * n + p];
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_permute_f32(const float* __restrict__ src,
+                                                     const uint32_t* __restrict__ perm, int n,
+                                                     float* __restrict__ dst) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
+    const uint32_t p = perm[i];
+    if (p < (uint32_t)n) dst[i] = src[p];
+  }
+}
+
 // ---- 6. octree ------------------------------------------------------------------------
 __device__ __forceinline__ int link_count(int w) { return w >> 4; }
 __device__ __forceinline__ int link_level(int w) { return w & 15; }
@@ -940,22 +965,11 @@ hipError_t iscan(const int* in, int* out, int* part, const int* level_off, int l
 
 }  // namespace
 
-int nb_bh_alloc(nb_ctx* c) {
+int nb_sort_alloc(nb_ctx* c) {
   nb_bh_state& b = c->bh;
+  if (b.keys[0]) return NB_OK;
   const int64_t n = c->n;
-  int64_t bound = 0, w = 1;
-  for (int l = 0; l <= kMaxLevel; ++l) {
-    bound += w < n ? w : n;
-    if (w < n) w *= 8;
-  }
-  b.max_nodes = bound;
   b.bbox_blocks = kBboxBlocks;
-  {
-    int dev = 0, cus = 256;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    b.persist_blocks = 4 * cus;  // the traversal's LDS allows four blocks per CU
-  }
   b.sort_blocks = (int)((n + kSortTile - 1) / kSortTile);
   const int64_t hist = 256 * (int64_t)b.sort_blocks;
   const int64_t scan_parts = (std::max<int64_t>(hist, n) + kScanTile - 1) / kScanTile + 2;
@@ -966,21 +980,44 @@ int nb_bh_alloc(nb_ctx* c) {
       A((void**)&b.vals[0], sizeof(uint32_t) * n) || A((void**)&b.vals[1], sizeof(uint32_t) * n) ||
       A((void**)&b.codes, sizeof(uint32_t) * n) ||
       A((void**)&b.digit_hist, sizeof(int) * 2 * hist) ||
-      A((void**)&b.pms, sizeof(float4) * (n + kSpare)) ||
+      A((void**)&b.iscan_part, sizeof(int) * scan_parts))
+    return nb_fail(NB_E_OOM, "device allocation failed (Morton sort)");
+  if (hipMemset(b.geom, 0, sizeof(float) * 32) != hipSuccess)
+    return nb_fail(NB_E_HIP, "hipMemset failed (Morton sort)");
+  return NB_OK;
+}
+
+int nb_bh_alloc(nb_ctx* c) {
+  nb_bh_state& b = c->bh;
+  const int64_t n = c->n;
+  int64_t bound = 0, w = 1;
+  for (int l = 0; l <= kMaxLevel; ++l) {
+    bound += w < n ? w : n;
+    if (w < n) w *= 8;
+  }
+  b.max_nodes = bound;
+  {
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    b.persist_blocks = 4 * cus;  // the traversal's LDS allows four blocks per CU
+  }
+  const int rs = nb_sort_alloc(c);
+  if (rs != NB_OK) return rs;
+  auto A = [](void** p, size_t bytes) { return hipMalloc(p, bytes < 256 ? 256 : bytes); };
+  if (A((void**)&b.pms, sizeof(float4) * (n + kSpare)) ||
       A((void**)&b.prefix, sizeof(double4) * (n + 1)) ||
       A((void**)&b.scan_part, sizeof(double4) * ((n + kScanTile - 1) / kScanTile + 2)) ||
       A((void**)&b.node_com, sizeof(float4) * bound) ||
       A((void**)&b.node_link, sizeof(int4) * bound) ||
       A((void**)&b.level_off, sizeof(int) * (kMaxLevel + 2)) ||
       A((void**)&b.child_cnt, sizeof(int) * n) || A((void**)&b.child_off, sizeof(int) * n) ||
-      A((void**)&b.iscan_part, sizeof(int) * scan_parts) ||
       A((void**)&b.counters, sizeof(int) * 16) ||
       A((void**)&b.waves, sizeof(int2) * n) ||
       A((void**)&b.inter, sizeof(unsigned long long) * 2))
     return nb_fail(NB_E_OOM, "nb_create: device allocation failed (octree)");
   // the batched leaf loop may read up to kSpare bodies past the end: zero mass
-  if (hipMemset(b.pms + n, 0, sizeof(float4) * kSpare) != hipSuccess ||
-      hipMemset(b.geom, 0, sizeof(float) * 32) != hipSuccess)
+  if (hipMemset(b.pms + n, 0, sizeof(float4) * kSpare) != hipSuccess)
     return nb_fail(NB_E_HIP, "nb_create: hipMemset failed (octree)");
   return NB_OK;
 }
@@ -996,7 +1033,7 @@ void nb_bh_free(nb_ctx* c) {
   b = nb_bh_state{};
 }
 
-hipError_t nb_bh_build(nb_ctx* c) {
+hipError_t nb_morton_sort(nb_ctx* c) {
   nb_bh_state& b = c->bh;
   const int n = c->n;
   hipStream_t s = c->stream;
@@ -1022,6 +1059,33 @@ hipError_t nb_bh_build(nb_ctx* c) {
                        b.vals[in], n, 8 * pass, nb, offs, b.keys[out], b.vals[out]);
   }
   b.sorted_buf = 0;
+  return hipGetLastError();
+}
+
+hipError_t nb_launch_permute_state(nb_ctx* c) {
+  nb_bh_state& b = c->bh;
+  const int n = c->n;
+  hipStream_t s = c->stream;
+  const int g256 = (n + 255) / 256;
+  const uint32_t* perm = b.vals[0];
+  double* arrays[4] = {c->pos, c->vel, c->acc, c->force};
+  for (double* a : arrays) {
+    hipLaunchKernelGGL(k_permute_d3, dim3(g256), dim3(256), 0, s, a, perm, n, c->tmp);
+    hipError_t e = hipMemcpyAsync(a, c->tmp, sizeof(double) * 3 * n, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) return e;
+  }
+  float* mtmp = reinterpret_cast<float*>(c->tmp);
+  hipLaunchKernelGGL(k_permute_f32, dim3(g256), dim3(256), 0, s, c->mass, perm, n, mtmp);
+  return hipMemcpyAsync(c->mass, mtmp, sizeof(float) * n, hipMemcpyDeviceToDevice, s);
+}
+
+hipError_t nb_bh_build(nb_ctx* c) {
+  nb_bh_state& b = c->bh;
+  const int n = c->n;
+  hipStream_t s = c->stream;
+  const int g256 = (n + 255) / 256;
+  hipError_t es = nb_morton_sort(c);
+  if (es != hipSuccess) return es;
   const uint32_t* codes_sorted = b.keys[0];
   const uint32_t* perm = b.vals[0];
   // 4. permute

@@ -40,6 +40,7 @@ void free_ctx(nb_ctx* c) {
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   nb_bh_free(c);
+  if (c->tmp) (void)hipFree(c->tmp);
   (void)hipFree(c->pos);
   (void)hipFree(c->vel);
   (void)hipFree(c->acc);
@@ -61,6 +62,18 @@ int check_flags(nb_ctx* c) {
   if (h & 4) return nb_fail(NB_E_TREE_OVERFLOW, "octree traversal left the node range");
   if (h & 8) return nb_fail(NB_E_TREE_OVERFLOW, "octree traversal stack overflow");
   return NB_OK;
+}
+
+
+// Structure-of-arrays host copy [k][slot] -> [k][caller index] after nb_reorder.
+template <typename T>
+void to_caller_order(const nb_ctx* c, std::vector<T>& a, int comps) {
+  if (c->order.empty()) return;
+  const size_t n = c->n;
+  std::vector<T> out(a.size());
+  for (int k = 0; k < comps; ++k)
+    for (size_t s = 0; s < n; ++s) out[k * n + (size_t)c->order[s]] = a[k * n + s];
+  a.swap(out);
 }
 
 }  // namespace
@@ -223,6 +236,7 @@ static int upload_common(nb_ctx* c, const std::vector<double>& p, const std::vec
   NB_HIP(hipStreamSynchronize(c->stream));
   c->uploaded = true;
   c->tree_valid = false;
+  c->order.clear();  // slots are in the caller's order again
   return NB_OK;
 }
 
@@ -320,6 +334,7 @@ int nb_forces(nb_ctx* c, float* force4) {
   const size_t n = c->n;
   std::vector<double> f(3 * n);
   NB_HIP(hipMemcpy(f.data(), c->force, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+  to_caller_order(c, f, 3);
   for (size_t i = 0; i < n; ++i) {
     force4[4 * i] = (float)f[i];
     force4[4 * i + 1] = (float)f[n + i];
@@ -336,6 +351,7 @@ int nb_forces_f64(nb_ctx* c, double* force3) {
   const size_t n = c->n;
   std::vector<double> f(3 * n);
   NB_HIP(hipMemcpy(f.data(), c->force, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+  to_caller_order(c, f, 3);
   for (size_t i = 0; i < n; ++i)
     for (int k = 0; k < 3; ++k) force3[3 * i + k] = f[k * n + i];
   return NB_OK;
@@ -351,6 +367,9 @@ static int download_soa(nb_ctx* c, std::vector<double>& p, std::vector<double>& 
   NB_HIP(hipMemcpy(p.data(), c->pos, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
   NB_HIP(hipMemcpy(v.data(), c->vel, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
   NB_HIP(hipMemcpy(a.data(), c->acc, 3 * n * sizeof(double), hipMemcpyDeviceToHost));
+  to_caller_order(c, p, 3);
+  to_caller_order(c, v, 3);
+  to_caller_order(c, a, 3);
   return check_flags(c);
 }
 
@@ -419,6 +438,7 @@ int nb_save_snapshot(nb_ctx* c, const char* path) {
   if (rc) return rc;
   std::vector<float> m(c->n);
   NB_HIP(hipMemcpy(m.data(), c->mass, sizeof(float) * c->n, hipMemcpyDeviceToHost));
+  to_caller_order(c, m, 1);
   FILE* f = std::fopen(path, "wb");
   if (!f) return nb_fail(NB_E_ARG, std::string("nb_save_snapshot: cannot open ") + path);
   const int64_t n = c->n;
@@ -458,6 +478,61 @@ int nb_load_snapshot(nb_ctx* c, const char* path) {
   std::fclose(f);
   if (!ok) return nb_fail(NB_E_ARG, "nb_load_snapshot: truncated file");
   return upload_common(c, p, v, a, m.data());
+}
+
+static int ensure_tmp(nb_ctx* c) {
+  if (!c->tmp && hipMalloc(&c->tmp, sizeof(double) * 3 * (size_t)c->n) != hipSuccess) {
+    c->tmp = nullptr;
+    return nb_fail(NB_E_OOM, "device allocation failed (staging)");
+  }
+  return NB_OK;
+}
+
+int nb_reorder(nb_ctx* c) {
+  if (!c) return nb_fail(NB_E_ARG, "nb_reorder: null context");
+  if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_reorder: nothing uploaded");
+  int rc = nb_sort_alloc(c);
+  if (rc != NB_OK) return rc;
+  rc = ensure_tmp(c);
+  if (rc != NB_OK) return rc;
+  NB_HIP(nb_morton_sort(c));
+  NB_HIP(nb_launch_permute_state(c));
+  NB_HIP(nb_launch_refresh_pm(c));
+  const size_t n = c->n;
+  std::vector<uint32_t> perm(n);
+  NB_HIP(hipMemcpyAsync(perm.data(), c->bh.vals[0], sizeof(uint32_t) * n, hipMemcpyDeviceToHost,
+                        c->stream));
+  NB_HIP(hipStreamSynchronize(c->stream));
+  std::vector<int32_t> order(n);
+  for (size_t k = 0; k < n; ++k)
+    order[k] = c->order.empty() ? (int32_t)perm[k] : c->order[perm[k]];
+  c->order.swap(order);
+  c->tree_valid = false;
+  return check_flags(c);
+}
+
+int nb_render_view(nb_ctx* c, float* xyzs) {
+  if (!c || !xyzs) return nb_fail(NB_E_ARG, "nb_render_view: null argument");
+  if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_render_view: nothing uploaded");
+  int rc = ensure_tmp(c);
+  if (rc != NB_OK) return rc;
+  float4* view = reinterpret_cast<float4*>(c->tmp);  // 16 B of the 24 B per body staged
+  NB_HIP(nb_launch_render_view(c, view));
+  const size_t n = c->n;
+  std::vector<float> h(4 * n);
+  NB_HIP(hipMemcpyAsync(h.data(), view, sizeof(float4) * n, hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(hipStreamSynchronize(c->stream));
+  for (size_t s = 0; s < n; ++s) {
+    const size_t i = c->order.empty() ? s : (size_t)c->order[s];
+    for (int k = 0; k < 4; ++k) xyzs[4 * i + k] = h[4 * s + k];
+  }
+  return check_flags(c);
+}
+
+int nb_get_order(nb_ctx* c, int32_t* order) {
+  if (!c || !order) return nb_fail(NB_E_ARG, "nb_get_order: null argument");
+  for (int64_t k = 0; k < c->n; ++k) order[k] = c->order.empty() ? (int32_t)k : c->order[k];
+  return NB_OK;
 }
 
 int nb_set_timing(nb_ctx* c, int32_t enable) {

@@ -124,6 +124,19 @@ __global__ __launch_bounds__(256) void k_refresh_pm(int n, int n_pad, const doub
     pm[i] = make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
+// Render view: (x, y, z, |v|) per body, the inputs of the reference's point
+// shaders (vertexShader.glsl:3-9 reads positions and velocities;
+// fragmentShader.glsl colours by length(velocity)).
+__global__ __launch_bounds__(256) void k_render_view(int n, const double* __restrict__ pos,
+                                                     const double* __restrict__ vel,
+                                                     float4* __restrict__ out) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const double vx = vel[i], vy = vel[n + i], vz = vel[2 * n + i];
+  out[i] = make_float4((float)pos[i], (float)pos[n + i], (float)pos[2 * n + i],
+                       (float)sqrt(vx * vx + vy * vy + vz * vz));
+}
+
 __global__ __launch_bounds__(256) void k_fill_i32(int* __restrict__ p, int v, int64_t count) {
   const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (i < count) p[i] = v;
@@ -209,6 +222,12 @@ hipError_t nb_launch_integrate(nb_ctx* c) {
 hipError_t nb_launch_refresh_pm(nb_ctx* c) {
   hipLaunchKernelGGL(k_refresh_pm, dim3(c->n_pad / 256), dim3(256), 0, c->stream, c->n, c->n_pad,
                      c->pos, c->mass, c->pm);
+  return hipGetLastError();
+}
+
+hipError_t nb_launch_render_view(nb_ctx* c, float4* out) {
+  hipLaunchKernelGGL(k_render_view, dim3((c->n + 255) / 256), dim3(256), 0, c->stream, c->n, c->pos,
+                     c->vel, out);
   return hipGetLastError();
 }
 

@@ -5,6 +5,7 @@
 
 #include <cstdint>
 #include <string>
+#include <vector>
 
 #include "nbody.h"
 
@@ -82,6 +83,10 @@ struct nb_ctx {
   bool guard = false;
 
   nb_bh_state bh;
+  // Morton reordering (nb_reorder): order[slot] = the caller's index of the
+  // body held in that slot; empty while slots are in the caller's order.
+  std::vector<int32_t> order;
+  double* tmp = nullptr;     // [3][n] permutation staging, allocated on first use
   bool uploaded = false;
   bool tree_valid = false;
 
@@ -100,10 +105,15 @@ hipError_t nb_launch_direct(nb_ctx* c);  // pm -> force
 hipError_t nb_launch_integrate(nb_ctx* c);  // force + state -> state, pm
 hipError_t nb_launch_refresh_pm(nb_ctx* c);  // pos, mass -> pm
 hipError_t nb_launch_fill_i32(int* p, int v, int64_t count, hipStream_t s);
+hipError_t nb_launch_render_view(nb_ctx* c, float4* out);  // (x, y, z, |v|) per slot
 hipError_t nb_launch_energy(nb_ctx* c, double* ekin_p, double* epot_p);
 
 // Barnes-Hut pipeline
-int nb_bh_alloc(nb_ctx* c);
+int nb_sort_alloc(nb_ctx* c);  // bbox, Morton and radix-sort buffers only
+int nb_bh_alloc(nb_ctx* c);    // everything (calls nb_sort_alloc)
+hipError_t nb_morton_sort(nb_ctx* c);  // bbox, Morton codes, stable sort: perm in bh.vals[0]
+// slots of pos/vel/acc/force/mass take the bodies of bh.vals[0] (staging in c->tmp)
+hipError_t nb_launch_permute_state(nb_ctx* c);
 void nb_bh_free(nb_ctx* c);
 hipError_t nb_bh_build(nb_ctx* c);     // bbox, Morton, sort, permute, prefix, octree
 hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions);  // -> force

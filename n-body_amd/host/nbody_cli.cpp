@@ -13,8 +13,12 @@
 //   --out path   write the final state in the reference format (full precision)
 //   --snapshot path / --resume path   binary device snapshots
 //   --energy     print FP64 energy and momentum before and after
+//   --reorder    Morton-reorder the resident state once after upload (gpuSort)
+//   --frames K --frame-out prefix   every K steps write prefix_NNNNN.xyzs: the
+//                render consumer's float32 (x, y, z, |v|) per body (nb_render_view)
 // The CPU versions (-v 1, 2, 5, 6) live in the test oracle, not in this engine.
 // Unlike the reference (D12), -f does not stop the parsing of later flags.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -31,7 +35,7 @@ namespace {
 [[noreturn]] void usage(const char* argv0, const char* why) {
   std::fprintf(stderr, "%s\nusage: %s [-v 4|7] [-n N] [-i 3|5|8|9] [-t dt] [-s eps2] [-f file] "
                "[--steps K] [--seed S] [--theta T] [--leaf L] [--out file] [--snapshot file] "
-               "[--resume file] [--energy]\n", why, argv0);
+               "[--resume file] [--energy] [--reorder] [--frames K --frame-out prefix]\n", why, argv0);
   std::exit(EXIT_FAILURE);
 }
 
@@ -95,8 +99,9 @@ int main(int argc, char** argv) {
   int steps = 100, leaf = 0;
   float theta = 0.5f;
   uint64_t seed = 1;
-  bool energy = false;
-  std::string file, out, snap, resume;
+  bool energy = false, reorder = false;
+  int frames = 0;
+  std::string file, out, snap, resume, frame_out;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
     auto val = [&]() -> const char* {
@@ -117,9 +122,14 @@ int main(int argc, char** argv) {
     else if (a == "--snapshot") snap = val();
     else if (a == "--resume") resume = val();
     else if (a == "--energy") energy = true;
+    else if (a == "--reorder") reorder = true;
+    else if (a == "--frames") frames = std::atoi(val());
+    else if (a == "--frame-out") frame_out = val();
     else usage(argv[0], ("unknown argument " + a).c_str());
   }
   if (n <= 0) usage(argv[0], "Invalid number of particles");
+  if (frames < 0 || (frames > 0) != !frame_out.empty())
+    usage(argv[0], "--frames K and --frame-out prefix go together");
   if (version == 1 || version == 2 || version == 5 || version == 6)
     usage(argv[0], "CPU solvers (-v 1, 2, 5, 6) are not part of this engine");
   if (version != 3 && version != 4 && version != 7) usage(argv[0], "Invalid version");
@@ -141,11 +151,33 @@ int main(int argc, char** argv) {
     nb_ctx* ctx = solver->context();
     if (!resume.empty() && (rc = nb_load_snapshot(ctx, resume.c_str())) != NB_OK)
       throw std::runtime_error(nb_last_error());
+    if (reorder && (rc = nb_reorder(ctx)) != NB_OK) throw std::runtime_error(nb_last_error());
     double ek0 = 0, ep0 = 0, p0[3];
     if (energy && nb_energy(ctx, &ek0, &ep0, p0) == NB_OK)
       std::printf("E0 = %.17g (kin %.17g, pot %.17g)\n", ek0 + ep0, ek0, ep0);
+    std::vector<float> view(frames > 0 ? 4 * (size_t)system.size() : 0);
+    int frame_no = 0;
+    auto write_frame = [&]() {
+      if ((rc = nb_render_view(ctx, view.data())) != NB_OK) throw std::runtime_error(nb_last_error());
+      char name[32];
+      std::snprintf(name, sizeof(name), "_%05d.xyzs", frame_no++);
+      FILE* f = std::fopen((frame_out + name).c_str(), "wb");
+      if (!f || std::fwrite(view.data(), sizeof(float), view.size(), f) != view.size()) {
+        if (f) std::fclose(f);
+        throw std::runtime_error("cannot write frame " + frame_out + name);
+      }
+      std::fclose(f);
+    };
     const auto t0 = std::chrono::steady_clock::now();
-    solver->updateParticlePositions(&system, steps);
+    if (frames > 0) {
+      write_frame();
+      for (int done = 0; done < steps; done += frames) {
+        solver->updateParticlePositions(&system, std::min(frames, steps - done));
+        write_frame();
+      }
+    } else {
+      solver->updateParticlePositions(&system, steps);
+    }
     if ((rc = nb_sync(ctx)) != NB_OK) throw std::runtime_error(nb_last_error());
     const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::printf("%d steps in %.3f s (%.3f ms/step)\n", steps, s, 1e3 * s / (steps ? steps : 1));

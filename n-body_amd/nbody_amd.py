@@ -190,6 +190,9 @@ _SIGS = {
     "nb_load_snapshot": (C.c_int, [_P, C.c_char_p]),
     "nb_debug_morton": (C.c_int, [_P, _P, _P]),
     "nb_debug_tree": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int64), _P]),
+    "nb_reorder": (C.c_int, [_P]),
+    "nb_render_view": (C.c_int, [_P, _P]),
+    "nb_get_order": (C.c_int, [_P, _P]),
     "nb_set_timing": (C.c_int, [_P, C.c_int32]),
     "nb_get_timing": (C.c_int, [_P, C.POINTER(Timing)]),
     "nb_count_interactions": (C.c_int, [_P, C.POINTER(C.c_int64)]),
@@ -414,6 +417,22 @@ class Simulation:
 
     def step(self, nsteps: int = 1):
         _check(self._lib.nb_step(self._ctx, nsteps))
+
+    def reorder(self):
+        """Permute the device state into Morton order (nb_reorder); the caller's
+        body order is kept at every copy."""
+        _check(self._lib.nb_reorder(self._ctx))
+
+    def render_view(self) -> np.ndarray:
+        """(x, y, z, |v|) per body, float32 (nb_render_view)."""
+        out = np.empty((self.n, 4), np.float32)
+        _check(self._lib.nb_render_view(self._ctx, _ptr(out)))
+        return out
+
+    def order(self) -> np.ndarray:
+        o = np.empty(self.n, np.int32)
+        _check(self._lib.nb_get_order(self._ctx, _ptr(o)))
+        return o
 
     def sync(self):
         _check(self._lib.nb_sync(self._ctx))

@@ -67,3 +67,22 @@ def test_cli_snapshot_resume(tmp_path):
     run_cli("-v", 4, "-i", 8, "-n", 3000, "--steps", 4, "--resume", snap, "--out", a)
     run_cli("-v", 4, "-i", 8, "-n", 3000, "--steps", 8, "--out", b)
     assert a.read_text() == b.read_text()
+
+
+def test_cli_frames_and_reorder(nb, tmp_path):
+    # frames of the render consumer, with and without a Morton reorder: the same
+    # bodies in the caller's order, and the last frame is the final state
+    prefix = tmp_path / "fr"
+    out = tmp_path / "final.txt"
+    run_cli("-v", 7, "-i", 8, "-n", 3000, "-t", 0.0035, "-s", 40, "--steps", 6, "--seed", 5,
+            "--frames", 3, "--frame-out", prefix, "--reorder", "--out", out)
+    frames = sorted(tmp_path.glob("fr_*.xyzs"))
+    assert [f.name for f in frames] == ["fr_00000.xyzs", "fr_00001.xyzs", "fr_00002.xyzs"]
+    first = np.fromfile(frames[0], np.float32).reshape(-1, 4)
+    last = np.fromfile(frames[-1], np.float32).reshape(-1, 4)
+    pos, vel, _ = nb.ic_plummer(5, 3000, 50.0, 1e4)
+    np.testing.assert_array_equal(first[:, :3], pos.astype(np.float32))
+    p, v, _, _ = nb.read_particle_file(out)
+    np.testing.assert_array_equal(last[:, :3], p[:, :3].astype(np.float32))
+    np.testing.assert_allclose(last[:, 3], np.linalg.norm(v[:, :3].astype(np.float64), axis=1),
+                               rtol=1e-6)
