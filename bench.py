@@ -194,13 +194,17 @@ def c4_clusters(nb, count=512, seed=4000):
     return out
 
 
-def bench_batch(nb, steps: int, warmup: int, energy: bool):
+def bench_batch(nb, steps: int, warmup: int, energy: bool, tune: bool = True):
     t0 = time.perf_counter()
     cl = c4_clusters(nb)
     log(f"batch: 512 clusters generated ({time.perf_counter() - t0:.1f} s)")
     total = sum(len(c[2]) for c in cl)
     with nb.Batch(sum((len(c[2]) + 255) // 256 * 256 for c in cl), len(cl)) as b:
         ids = [b.add(p, v, m, e, d) for p, v, m, e, d in cl]
+        shape = b.tune(2) if tune else None
+        if shape:
+            log(f"batch: tuned shape {shape['bodies_per_lane']}x{shape['j_split']} "
+                f"({shape['direct_ms']:.2f} ms force pass)")
         e0 = [b.energy(i) for i in ids] if energy else None
         _, bodies, inter = b.info()
         b.step(warmup)
@@ -220,6 +224,8 @@ def bench_batch(nb, steps: int, warmup: int, energy: bool):
             "ms_per_step": 1e3 * wall / steps,
             "force_ms": t["force_ms"] / max(1, t["force_calls"]),
             "interactions_per_s": inter * steps / wall,
+            "shape": {"bodies_per_lane": shape["bodies_per_lane"], "j_split": shape["j_split"],
+                      "tuned": True} if shape else "built-in 2x8",
         }
         if energy:
             e1 = [b.energy(i) for i in ids]
@@ -430,7 +436,8 @@ def main():
         print(json.dumps(bench_bh(nb, args.bh_steps, 2)), flush=True)
         return
     if args.only == "batch":
-        print(json.dumps(bench_batch(nb, args.batch_steps, 2, args.batch_energy)), flush=True)
+        print(json.dumps(bench_batch(nb, args.batch_steps, 2, args.batch_energy, not args.no_tune)),
+              flush=True)
         return
     if args.only == "direct":
         args.no_bh = args.no_batch = args.no_cpu = True
@@ -500,7 +507,7 @@ def main():
         log(f"bh: {result['bh_4m']['ms_per_step']:.2f} ms/step")
     if not args.no_batch:
         log("batch: C4 ensemble")
-        result["batch_c4"] = bench_batch(nb, args.batch_steps, 2, args.batch_energy)
+        result["batch_c4"] = bench_batch(nb, args.batch_steps, 2, args.batch_energy, not args.no_tune)
         log(f"batch: {result['batch_c4']['ms_per_step']:.2f} ms/step")
     if args.serve > 0:
         result["serve"] = bench_serve(nb, args.serve_rate, args.serve, args.serve)

@@ -71,7 +71,8 @@ typedef struct nb_config {
   int32_t integrator; /* nb_integrator */
   int32_t leaf_size;  /* BH bodies per leaf, 1..64; 0 = tuned default */
   int32_t bodies_per_lane; /* direct: i-bodies per lane (1,2,4); 0 = tuned default */
-  int32_t j_split;    /* direct: waves sharing one i-block (1,2,4); 0 = tuned default */
+  int32_t j_split;    /* direct: waves sharing one i-block (1,2,4; 8 with bodies_per_lane
+                         <= 2; 16 with 1); 0 = tuned default */
   int32_t flags;      /* NB_F_* */
 } nb_config;
 
@@ -199,8 +200,8 @@ typedef struct nb_batch_config {
   int64_t max_bodies;      /* body pool capacity */
   int32_t max_clusters;
   int32_t integrator;      /* nb_integrator, shared by all clusters */
-  int32_t bodies_per_lane; /* 0 = tuned default */
-  int32_t j_split;         /* 0 = tuned default */
+  int32_t bodies_per_lane; /* 0 = built-in 2 (nb_batch_tune picks per ensemble) */
+  int32_t j_split;         /* 0 = built-in 8 (4 with bodies_per_lane 4) */
   int32_t flags;           /* NB_F_NO_GRAPH */
   int32_t pad_;
 } nb_batch_config;
@@ -292,7 +293,8 @@ double nb_server_now(nb_server* s);
  * sphere of n bodies and installs the fastest as the default of later
  * nb_create / nb_batch_create calls for systems in the same power-of-two size
  * class (explicit non-zero config fields still win). Candidates: direct sum
- * bodies_per_lane x j_split in {1,2,4} x {1,2,4}; Barnes-Hut leaf_size in
+ * bodies_per_lane x j_split in {1,2,4} x {1,2,4,8,16} (the shapes a kernel
+ * exists for: 8 needs bodies_per_lane <= 2, 16 needs 1); Barnes-Hut leaf_size in
  * {1,2,4,8,16,32}. The reference hard-codes its work-group sizes (64, 320,
  * 1024) in the shaders. */
 typedef struct nb_tuning {
@@ -307,6 +309,11 @@ typedef struct nb_tuning {
 #define NB_TUNE_DIRECT 0x1
 #define NB_TUNE_BH 0x2
 int nb_autotune(int64_t n, int32_t what, int32_t reps, nb_tuning* out);
+/* Batched ensembles: time every valid bodies_per_lane x j_split shape of the
+ * persistent force pass on the batch's current clusters (reps passes each; the
+ * particle state is not changed) and keep the fastest for later steps.
+ * out->direct_ms = its force pass, out->n = bodies in the batch. */
+int nb_batch_tune(nb_batch* b, int32_t reps, nb_tuning* out);
 /* The defaults nb_create would use for n bodies (tuned or built in). */
 int nb_get_tuning(int64_t n, nb_tuning* out);
 

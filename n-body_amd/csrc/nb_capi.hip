@@ -78,13 +78,29 @@ void to_caller_order(const nb_ctx* c, std::vector<T>& a, int comps) {
 
 }  // namespace
 
-// Built-in launch shapes (measured on MI355X at N = 2^18: one body per lane and
-// four waves per block, 20.3 ms against 24.4 ms for two and two).
+// Built-in launch shapes, from a sweep on MI355X (Plummer spheres, force pass):
+// one body per lane; 16 waves per i-block up to 16384 bodies (4096 bodies:
+// 30 us against 83 us at 4 waves), 8 above (2^15: 0.41 ms against 0.61 ms;
+// 2^18: 19.8 ms against 20.2 ms).
 void nb_default_shape(int64_t n, int* bpl, int* jsplit, int* leaf) {
-  (void)n;
   if (bpl && *bpl == 0) *bpl = 1;
-  if (jsplit && *jsplit == 0) *jsplit = 4;
+  if (jsplit && *jsplit == 0) {
+    int k = n <= 16384 ? 16 : 8;
+    if (bpl && *bpl == 2) k = 8;
+    if (bpl && *bpl == 4) k = 4;
+    *jsplit = k;
+  }
   if (leaf && *leaf == 0) *leaf = 8;
+}
+
+// Valid direct-sum launch shapes (see launch_b in nb_direct.hip).
+bool nb_valid_shape(int bpl, int jsplit) {
+  switch (bpl) {
+    case 1: return jsplit == 1 || jsplit == 2 || jsplit == 4 || jsplit == 8 || jsplit == 16;
+    case 2: return jsplit == 1 || jsplit == 2 || jsplit == 4 || jsplit == 8;
+    case 4: return jsplit == 1 || jsplit == 2 || jsplit == 4;
+    default: return false;
+  }
 }
 
 namespace {
@@ -93,6 +109,7 @@ void default_direct_shape(nb_ctx* c) {
   int bpl = c->cfg.bodies_per_lane, js = c->cfg.j_split, leaf = c->cfg.leaf_size;
   nb_tuned_shape(c->cfg.n, &bpl, &js, &leaf);
   nb_default_shape(c->cfg.n, &bpl, &js, &leaf);
+  if (!nb_valid_shape(bpl, js)) js = std::min(js, bpl == 1 ? 16 : bpl == 2 ? 8 : 4);
   c->bpl = bpl;
   c->jsplit = js;
   c->cfg.leaf_size = leaf;
@@ -158,8 +175,12 @@ int nb_create(nb_ctx** out, const nb_config* cfg) {
   if (cfg->bodies_per_lane != 0 && cfg->bodies_per_lane != 1 && cfg->bodies_per_lane != 2 &&
       cfg->bodies_per_lane != 4)
     return nb_fail(NB_E_ARG, "nb_create: bodies_per_lane must be 0, 1, 2 or 4");
-  if (cfg->j_split != 0 && cfg->j_split != 1 && cfg->j_split != 2 && cfg->j_split != 4)
-    return nb_fail(NB_E_ARG, "nb_create: j_split must be 0, 1, 2 or 4");
+  if (cfg->j_split != 0 && cfg->j_split != 1 && cfg->j_split != 2 && cfg->j_split != 4 &&
+      cfg->j_split != 8 && cfg->j_split != 16)
+    return nb_fail(NB_E_ARG, "nb_create: j_split must be 0, 1, 2, 4, 8 or 16");
+  if (cfg->bodies_per_lane != 0 && cfg->j_split != 0 &&
+      !nb_valid_shape(cfg->bodies_per_lane, cfg->j_split))
+    return nb_fail(NB_E_ARG, "nb_create: j_split 8 needs bodies_per_lane <= 2, 16 needs 1");
   if (cfg->theta < 0.f || !std::isfinite(cfg->theta))
     return nb_fail(NB_E_ARG, "nb_create: theta must be finite and >= 0");
 

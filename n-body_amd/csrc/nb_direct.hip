@@ -62,12 +62,20 @@ hipError_t launch_bk(const nb_ctx* c) {
   return hipGetLastError();
 }
 
+// Shapes: B in {1, 2, 4}; K in {1, 2, 4}, plus K = 8 for B <= 2 and K = 16 for
+// B = 1 (small systems need the extra waves; the LDS merge holds K*6*B*64 floats).
 template <int B>
 hipError_t launch_b(const nb_ctx* c) {
   switch (c->jsplit) {
     case 1: return launch_bk<B, 1>(c);
     case 2: return launch_bk<B, 2>(c);
     case 4: return launch_bk<B, 4>(c);
+    case 8:
+      if constexpr (B <= 2) return launch_bk<B, 8>(c);
+      return hipErrorInvalidValue;
+    case 16:
+      if constexpr (B == 1) return launch_bk<B, 16>(c);
+      return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
 }

@@ -122,6 +122,7 @@ hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions);  // -> force
 // from the built-in defaults.
 bool nb_tuned_shape(int64_t n, int* bpl, int* jsplit, int* leaf);
 void nb_default_shape(int64_t n, int* bpl, int* jsplit, int* leaf);
+bool nb_valid_shape(int bpl, int jsplit);  // a direct-sum kernel exists for it
 
 // step = forces + integration (no sync)
 hipError_t nb_enqueue_step(nb_ctx* c);

@@ -99,7 +99,8 @@ extern "C" int nb_autotune(int64_t n, int32_t what, int32_t reps, nb_tuning* out
   if (what & NB_TUNE_DIRECT) {
     double best_ms = DBL_MAX;
     for (int b : {1, 2, 4})
-      for (int k : {1, 2, 4}) {
+      for (int k : {1, 2, 4, 8, 16}) {
+        if (!nb_valid_shape(b, k)) continue;
         nb_config cfg{};
         cfg.n = n;
         cfg.eps2 = 40.f;

@@ -208,6 +208,7 @@ _SIGS = {
     "nb_batch_download": (C.c_int, [_P, C.c_int32, _P, _P, _P]),
     "nb_batch_energy": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "nb_batch_info": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_int64)]),
+    "nb_batch_tune": (C.c_int, [_P, C.c_int32, C.POINTER(Tuning)]),
     "nb_batch_set_timing": (C.c_int, [_P, C.c_int32]),
     "nb_batch_get_timing": (C.c_int, [_P, C.POINTER(Timing)]),
     "nb_server_create": (C.c_int, [C.POINTER(_P), C.POINTER(ServerConfig)]),
@@ -557,6 +558,12 @@ class Batch:
 
     def step(self, nsteps: int = 1):
         _check(self._lib.nb_batch_step(self._b, nsteps))
+
+    def tune(self, reps: int = 3) -> dict:
+        """Time every launch shape on the current clusters and keep the fastest."""
+        t = Tuning()
+        _check(self._lib.nb_batch_tune(self._b, reps, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Tuning._fields_ if k != "pad_"}
 
     def sync(self):
         _check(self._lib.nb_batch_sync(self._b))

@@ -51,7 +51,9 @@ def test_argument_errors(nb):
     lib = nb.load()
     ctx = ctypes.c_void_p()
     bad = [dict(n=0), dict(n=1 << 30), dict(n=8, eps2=-1.0), dict(n=8, leaf_size=65),
-           dict(n=8, bodies_per_lane=3), dict(n=8, j_split=8), dict(n=8, solver=7),
+           dict(n=8, bodies_per_lane=3), dict(n=8, j_split=32), dict(n=8, j_split=3),
+           dict(n=8, bodies_per_lane=4, j_split=8), dict(n=8, bodies_per_lane=2, j_split=16),
+           dict(n=8, solver=7),
            dict(n=1 << 27, solver=1)]
     for kw in bad:
         cfg = nb.Config(**{"eps2": 1.0, "dt": 0.01, **kw})
@@ -110,3 +112,10 @@ def test_sizes_and_uniform(nb):
     assert 4000 < np.mean(s) < 9000  # E = (32768 - 256) / ln(128) ~ 6701
     u = nb.uniform(2, 1000, 1.0, 40.0)
     assert u.min() >= 1.0 and u.max() < 40.0
+
+
+def test_default_shape_fills_small_systems(nb):
+    # K = 16 waves per i-block up to 16384 bodies, 8 above
+    for n, k in ((1024, 16), (4096, 16), (16384, 16), (16385, 8), (65536, 8), (1 << 18, 8)):
+        t = nb.get_tuning(n)
+        assert (t["bodies_per_lane"], t["j_split"]) == (1, k), (n, t)

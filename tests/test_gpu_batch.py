@@ -93,7 +93,7 @@ def test_batch_graph_equals_eager(nb):
             assert np.array_equal(x, y)
 
 
-@pytest.mark.parametrize("bpl,js", [(1, 1), (2, 2), (4, 4)])
+@pytest.mark.parametrize("bpl,js", [(1, 1), (2, 2), (4, 4), (1, 16), (2, 8)])
 def test_batch_launch_shapes(nb, ora, bpl, js):
     cl = make_clusters(nb, 5, 256, 1500, 41 + bpl)
     with nb.Batch(sum(len(c[2]) + 256 for c in cl), 8, bodies_per_lane=bpl, j_split=js) as b:
@@ -101,3 +101,25 @@ def test_batch_launch_shapes(nb, ora, bpl, js):
         for cid, (p, v, m, e, d) in zip(ids, cl):
             ref = ora.direct_f64(p.astype(np.float32).astype(np.float64), m.astype(np.float64), e)
             assert np.percentile(rel_err(b.forces(cid), ref), 99) < 1e-5
+
+
+def test_tune_picks_a_shape_and_keeps_results(nb, ora):
+    cl = make_clusters(nb, 10, 256, 4000, 17)
+    with nb.Batch(sum(len(c[2]) + 256 for c in cl), 16) as b:
+        ids = [b.add(p, v, m, e, d) for p, v, m, e, d in cl]
+        p0 = [b.download(cid)[0] for cid in ids]
+        t = b.tune(reps=2)
+        assert (t["bodies_per_lane"], t["j_split"]) in {(bb, k) for bb in (1, 2, 4) for k in (1, 2, 4)} | \
+            {(1, 8), (1, 16), (2, 8)}
+        assert t["direct_ms"] > 0 and t["n"] == sum(len(c[2]) for c in cl)
+        # timing the shapes did not move any body
+        for cid, p in zip(ids, p0):
+            assert np.array_equal(b.download(cid)[0], p)
+        b.step(3)
+        for cid, (p, v, m, e, d) in zip(ids, cl):
+            gp, _, _ = b.download(cid)
+            op, ov, oa = p.copy(), v.copy(), np.zeros_like(p)
+            for _ in range(3):
+                f = ora.direct_f64(op.astype(np.float32).astype(np.float64), m.astype(np.float64), e)
+                ora.kdk_update_f64(op, ov, oa, f, d)
+            assert np.abs(gp - op).max() < 1e-9 * max(1.0, np.abs(op).max())

@@ -92,7 +92,8 @@ def test_c2_size_fp64_subsample(nb, ora):
     assert np.percentile(e, 99) < 1e-4
 
 
-@pytest.mark.parametrize("bpl,js", [(1, 1), (1, 4), (2, 2), (4, 1), (4, 4)])
+@pytest.mark.parametrize("bpl,js", [(1, 1), (1, 4), (1, 8), (1, 16), (2, 2), (2, 8), (4, 1),
+                                    (4, 4)])
 def test_launch_shapes_agree(nb, ora, bpl, js):
     n = 3000
     pos, vel, m = nb.ic_plummer(5, n, 20.0, 100.0)
@@ -197,7 +198,8 @@ def test_timing_counters(nb):
 def test_autotune_installs_defaults(nb, ora):
     before = nb.get_tuning(20000)
     t = nb.autotune(20000, nb.TUNE_DIRECT | nb.TUNE_BH, reps=2)
-    assert t["bodies_per_lane"] in (1, 2, 4) and t["j_split"] in (1, 2, 4)
+    assert (t["bodies_per_lane"], t["j_split"]) in {(b, k) for b in (1, 2, 4) for k in (1, 2, 4)} | \
+        {(1, 8), (1, 16), (2, 8)}
     assert t["leaf_size"] in (1, 2, 4, 8, 16, 32)
     assert t["direct_ms"] > 0 and t["bh_ms"] > 0
     after = nb.get_tuning(30000)  # same power-of-two class
@@ -212,3 +214,4 @@ def test_autotune_installs_defaults(nb, ora):
     ref = ora.direct_f64(pos.astype(np.float32).astype(np.float64), m.astype(np.float64), 4.0,
                          targets=tgt)
     assert np.percentile(rel_err(f[tgt], ref), 99) < 1e-5
+
