@@ -730,10 +730,14 @@ constexpr uint32_t kCellBit = 0x80000000u;    // list entry refers to a cell, no
 
 // Stream the list through the direct-sum inner loop: 64 entries at a time,
 // each lane gathers one source (cell COM or body) into the LDS staging area,
-// then every lane reads the 64 staged sources by broadcast.
+// then the lanes read the staged sources by broadcast. The bodies of a range are
+// held S = 64 / P times (P = its body count rounded up to a power of two, at
+// least 4; S = 1 above 32 bodies): the lanes of slice s = lane / P take the staged entries
+// s, s + S, s + 2S, ..., so every lane does useful work, and the slices' partial
+// sums are added at the end of the range.
 template <bool kCount, bool kGuard>
 __device__ __forceinline__ void list_flush(const uint32_t* list, int lc, float4* stage, int lane,
-                                           const float4* __restrict__ pms,
+                                           int slice, int S, const float4* __restrict__ pms,
                                            const float4* __restrict__ com, float4 me, float eps2,
                                            float& ax, float& ay, float& az,
                                            unsigned long long& cnt) {
@@ -746,10 +750,14 @@ __device__ __forceinline__ void list_flush(const uint32_t* list, int lc, float4*
     }
     stage[lane] = q;
     __builtin_amdgcn_wave_barrier();
-    for (int j = 0; j < m; j += 4) {
+    // entries slice + S*t for t < T, T = ceil(m / S) rounded up to 4: S*T <= 64
+    // because S divides 64 and S <= 16, so every read stays inside the 64 staged
+    // entries; entries at or past m are zero-mass pads.
+    const int T = ((m + S - 1) / S + 3) & ~3;
+    for (int t = 0; t < T; t += 4) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const float4 src = stage[j + u];  // same address on every lane: LDS broadcast
+        const float4 src = stage[slice + S * (t + u)];  // few distinct addresses: LDS broadcast
         const float dx = src.x - me.x, dy = src.y - me.y, dz = src.z - me.z;
         const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
         const float inv = __builtin_amdgcn_rsqf(r2);
@@ -777,12 +785,13 @@ struct WaveLds {
 // Walk the tree for the bodies of one wave range (lanes past the range hold a
 // copy of a valid body) and accumulate their accelerations.
 template <bool kCount, bool kGuard>
-__device__ void traverse_range(const WaveLds& L, int lane, float4 me, int n,
+__device__ void traverse_range(const WaveLds& L, int lane, int P, float4 me, int n,
                                const float4* __restrict__ pms, const float4* __restrict__ com,
                                const int4* __restrict__ link, float theta2, float eps2,
                                int* __restrict__ flags, float& ax, float& ay, float& az,
                                unsigned long long& cnt) {
   const uint64_t lt = (uint64_t(1) << lane) - 1;
+  const int S = 64 / P, slice = lane / P;
   float lx = me.x, ly = me.y, lz = me.z, hx = me.x, hy = me.y, hz = me.z;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -797,7 +806,8 @@ __device__ void traverse_range(const WaveLds& L, int lane, float4 me, int n,
   uint32_t* list = L.list;
 #define NB_FLUSH()                                                                              \
   do {                                                                                          \
-    list_flush<kCount, kGuard>(list, lc, L.stage, lane, pms, com, me, eps2, ax, ay, az, cnt);  \
+    list_flush<kCount, kGuard>(list, lc, L.stage, lane, slice, S, pms, com, me, eps2, ax, ay, \
+                               az, cnt);                                                       \
     lc = 0;                                                                                     \
   } while (0)
   // a virtual parent whose only child is the root
@@ -910,21 +920,30 @@ __global__ __launch_bounds__(64 * kListWaves) void k_traverse_list(
     w = __shfl(w, 0);
     if (w >= nranges) break;
     const int2 range = waves[w];
-    const bool valid = lane < range.y;
-    const int t = min(range.x + (valid ? lane : 0), n - 1);
+    // P lanes per copy of the range's bodies: a power of two >= its count, >= 4
+    const int P = range.y <= 4 ? 4 : 1 << (32 - __clz(range.y - 1));
+    const int slot = lane & (P - 1);
+    const bool valid = slot < range.y;
+    const int t = min(range.x + (valid ? slot : 0), n - 1);
     const float4 me = pms[t];
     float ax = 0.f, ay = 0.f, az = 0.f;
     unsigned long long c = 0;
-    traverse_range<kCount, kGuard>(L, lane, me, n, pms, com, link, theta2, eps2, flags, ax, ay,
-                                   az, c);
-    if (valid) {
+    traverse_range<kCount, kGuard>(L, lane, P, me, n, pms, com, link, theta2, eps2, flags, ax,
+                                   ay, az, c);
+    // add the slices' partial sums (butterfly over the slice bits, fixed order)
+    for (int off = P; off < 64; off <<= 1) {
+      ax += __shfl_xor(ax, off);
+      ay += __shfl_xor(ay, off);
+      az += __shfl_xor(az, off);
+    }
+    if (valid) cnt += c;
+    if (valid && lane < P) {
       const uint32_t o = perm[t];
       if (o < (uint32_t)n) {
         force[o] = G * (double)ax;
         force[n + o] = G * (double)ay;
         force[2 * n + o] = G * (double)az;
       }
-      cnt += c;
     }
   }
   if (kCount) {
