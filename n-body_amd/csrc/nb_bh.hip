@@ -724,7 +724,10 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ pms
 // gathers the sources 64 at a time into LDS and streams them through the
 // direct-sum inner loop (LDS broadcast reads, 13 VALU per interaction).
 constexpr int kListWaves = 4;
-constexpr int kStackCap = 768;
+constexpr int kListBlocksPerCU = 5;  // LDS (28.8 KB per block) and 96 VGPRs allow five
+// The deepest stack seen on the two-disc 4M system (400 sampled ranges, leaf 8)
+// is 73 entries; overflow is detected and reported (NB_E_TREE_OVERFLOW).
+constexpr int kStackCap = 512;
 constexpr int kListCap = 512;                 // source indices per wave
 constexpr uint32_t kCellBit = 0x80000000u;    // list entry refers to a cell, not a body
 
@@ -893,7 +896,7 @@ __device__ void traverse_range(const WaveLds& L, int lane, int P, float4 me, int
 // Persistent: every wave takes wave ranges from a ticket until none is left,
 // so the cost of uneven ranges balances dynamically.
 template <bool kCount, bool kGuard>
-__global__ __launch_bounds__(64 * kListWaves) void k_traverse_list(
+__global__ __launch_bounds__(64 * kListWaves, kListBlocksPerCU) void k_traverse_list(
     const float4* __restrict__ pms, const uint32_t* __restrict__ perm, int n,
     const float4* __restrict__ com, const int4* __restrict__ link,
     const int* __restrict__ level_off, int64_t max_nodes, const float* __restrict__ geom,
@@ -1019,7 +1022,7 @@ int nb_bh_alloc(nb_ctx* c) {
     int dev = 0, cus = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    b.persist_blocks = 4 * cus;  // the traversal's LDS allows four blocks per CU
+    b.persist_blocks = kListBlocksPerCU * cus;  // resident traversal blocks
   }
   const int rs = nb_sort_alloc(c);
   if (rs != NB_OK) return rs;
