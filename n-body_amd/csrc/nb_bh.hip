@@ -483,19 +483,6 @@ __global__ __launch_bounds__(256) void k_permute_f32(const float* __restrict__ s
 __device__ __forceinline__ int link_count(int w) { return w >> 4; }
 __device__ __forceinline__ int link_level(int w) { return w & 15; }
 
-__global__ void k_tree_init(int n, int4* __restrict__ link, int* __restrict__ level_off,
-                            int2* __restrict__ waves, int* __restrict__ wctl) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  link[0] = make_int4(-1, -1, 0, n << 4);
-  // wave ranges: a system of at most kWaveBodies bodies is one range
-  wctl[0] = n <= kWaveBodies ? 1 : 0;  // range count
-  wctl[1] = 0;                         // traversal ticket
-  if (n <= kWaveBodies) waves[0] = make_int2(0, n);
-  level_off[0] = 0;
-  level_off[1] = 1;
-  for (int l = 2; l <= kMaxLevel + 1; ++l) level_off[l] = 1;
-}
-
 // First index in [lo, hi) whose digit (codes >> shift) & 7 is >= d (digits are sorted).
 __device__ __forceinline__ int digit_lower_bound(const uint32_t* __restrict__ codes, int lo,
                                                  int hi, int shift, uint32_t d) {
@@ -505,6 +492,22 @@ __device__ __forceinline__ int digit_lower_bound(const uint32_t* __restrict__ co
     else hi = mid;
   }
   return lo;
+}
+
+// Boundaries of the 8 digit cells of the sorted range [s, e): bound[d] = first
+// index whose digit is >= d (bound[0] = s, bound[8] = e), by binary searches.
+// (Counting the digits of short ranges with independent loads instead was
+// measured slower at every level on the 4M two-disc system.)
+__device__ __forceinline__ void digit_bounds(const uint32_t* __restrict__ codes, int s, int e,
+                                             int shift, int (&bound)[9]) {
+  bound[0] = s;
+  bound[8] = e;
+  int b = s;
+#pragma unroll
+  for (int d = 1; d < 8; ++d) {
+    b = digit_lower_bound(codes, b, e, shift, (uint32_t)d);
+    bound[d] = b;
+  }
 }
 
 __global__ __launch_bounds__(256) void k_tree_count(const uint32_t* __restrict__ codes, int n,
@@ -520,16 +523,39 @@ __global__ __launch_bounds__(256) void k_tree_count(const uint32_t* __restrict__
     const int cnt = link_count(nd.w);
     int nc = 0;
     if (cnt > leaf && nd.z >= 0 && nd.z + cnt <= n) {
-      const int s = nd.z, e = nd.z + cnt;
-      int b = s;
-      for (uint32_t d = 1; d <= 8; ++d) {
-        const int c = d < 8 ? digit_lower_bound(codes, b, e, shift, d) : e;
-        nc += c > b;
-        b = c;
-      }
+      int bound[9];
+      digit_bounds(codes, nd.z, nd.z + cnt, shift, bound);
+#pragma unroll
+      for (int d = 0; d < 8; ++d) nc += bound[d + 1] > bound[d] ? 1 : 0;
     }
     child_cnt[k - lo] = nc;
   }
+}
+
+// Wave ranges of one node's children (see kWaveBodies): calls out(start, count)
+// for each range in order; returns how many there are.
+template <typename Out>
+__device__ __forceinline__ int pack_ranges(const int* starts, const int* ends, int nc, int l,
+                                           int leaf, Out out) {
+  int r = 0, run_s = 0, run_n = 0;
+  for (int q = 0; q < nc; ++q) {
+    const int cn = ends[q] - starts[q];
+    if (cn > kWaveBodies) {
+      if (run_n) { out(r++, run_s, run_n); }
+      run_n = 0;
+      if (l + 1 == kMaxLevel || cn <= leaf)  // a leaf: chunks (internal: its own level)
+        for (int o = 0; o < cn; o += kWaveBodies) out(r++, starts[q] + o, min(kWaveBodies, cn - o));
+    } else {
+      if (run_n + cn > kWaveBodies) {
+        out(r++, run_s, run_n);
+        run_n = 0;
+      }
+      if (run_n == 0) run_s = starts[q];
+      run_n += cn;
+    }
+  }
+  if (run_n) out(r++, run_s, run_n);
+  return r;
 }
 
 __global__ __launch_bounds__(256) void k_tree_emit(const uint32_t* __restrict__ codes, int n,
@@ -543,59 +569,218 @@ __global__ __launch_bounds__(256) void k_tree_emit(const uint32_t* __restrict__ 
   const int lo = level_off[l];
   const int hi = min(level_off[l + 1], lo + cap);
   const int next_lo = level_off[l + 1];
-  auto emit = [&](int start, int count) {
-    const int slot = atomicAdd(&wctl[0], 1);
-    if (slot < n) waves[slot] = make_int2(start, count);  // at most n ranges exist
-  };
   const int shift = 27 - 3 * l;
-  for (int k = lo + blockIdx.x * 256 + threadIdx.x; k < hi; k += gridDim.x * 256) {
-    int4 nd = link[k];
-    const int cnt = link_count(nd.w);
-    if (cnt <= leaf || nd.z < 0 || nd.z + cnt > n) continue;
-    const int first = next_lo + child_off[k - lo];
-    const int s = nd.z, e = nd.z + cnt;
+  const int lane = threadIdx.x & 63;
+  // Rounds are uniform across the block, so every lane of a wave reaches the
+  // wave-wide range allocation below.
+  for (int base = lo + blockIdx.x * 256; base < hi; base += gridDim.x * 256) {
+    const int k = base + threadIdx.x;
     int starts[8], ends[8], nc = 0;
-    int b = s;
-    for (uint32_t d = 1; d <= 8; ++d) {
-      const int c = d < 8 ? digit_lower_bound(codes, b, e, shift, d) : e;
-      if (c > b) {
-        starts[nc] = b;
-        ends[nc] = c;
-        ++nc;
-      }
-      b = c;
+    int4 nd = make_int4(-1, -1, -1, 0);
+    bool internal = false;
+    if (k < hi) {
+      nd = link[k];
+      const int cnt = link_count(nd.w);
+      internal = cnt > leaf && nd.z >= 0 && nd.z + cnt <= n;
     }
-    if ((int64_t)first + nc > max_nodes) {
-      atomicOr(flags, 2);
-      continue;
-    }
-    for (int q = 0; q < nc; ++q) {
-      const int idx = first + q;
-      const int nxt = q + 1 < nc ? idx + 1 : nd.y;
-      link[idx] = make_int4(-1, nxt, starts[q], ((ends[q] - starts[q]) << 4) | (l + 1));
-    }
-    link[k].x = first;
-    if (cnt > kWaveBodies) {  // pack the children into wave ranges
-      int run_s = 0, run_n = 0;
-      for (int q = 0; q < nc; ++q) {
-        const int cn = ends[q] - starts[q];
-        if (cn > kWaveBodies) {
-          if (run_n) emit(run_s, run_n);
-          run_n = 0;
-          if (l + 1 == kMaxLevel || cn <= leaf)  // a leaf: chunks (internal: its own level)
-            for (int o = 0; o < cn; o += kWaveBodies) emit(starts[q] + o, min(kWaveBodies, cn - o));
-        } else {
-          if (run_n + cn > kWaveBodies) {
-            emit(run_s, run_n);
-            run_n = 0;
-          }
-          if (run_n == 0) run_s = starts[q];
-          run_n += cn;
+    if (internal) {
+      int bound[9];
+      digit_bounds(codes, nd.z, nd.z + link_count(nd.w), shift, bound);
+#pragma unroll
+      for (int d = 0; d < 8; ++d)
+        if (bound[d + 1] > bound[d]) {
+          starts[nc] = bound[d];
+          ends[nc] = bound[d + 1];
+          ++nc;
         }
+      const int first = next_lo + child_off[k - lo];
+      if ((int64_t)first + nc > max_nodes) {
+        atomicOr(flags, 2);
+        internal = false;
+      } else {
+        for (int q = 0; q < nc; ++q) {
+          const int idx = first + q;
+          const int nxt = q + 1 < nc ? idx + 1 : nd.y;
+          link[idx] = make_int4(-1, nxt, starts[q], ((ends[q] - starts[q]) << 4) | (l + 1));
+        }
+        link[k].x = first;
       }
-      if (run_n) emit(run_s, run_n);
+    }
+    // wave ranges of the children of nodes holding more than kWaveBodies bodies:
+    // one atomic per wave reserves the slots of all its lanes
+    const bool packs = internal && link_count(nd.w) > kWaveBodies;
+    const int nr = packs ? pack_ranges(starts, ends, nc, l, leaf, [](int, int, int) {}) : 0;
+    int incl = nr;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int v = __shfl_up(incl, off);
+      if (lane >= off) incl += v;
+    }
+    const int total = __shfl(incl, 63);
+    int wbase = 0;
+    if (lane == 63 && total > 0) wbase = atomicAdd(&wctl[0], total);
+    wbase = __shfl(wbase, 63);
+    if (nr > 0) {
+      const int first_slot = wbase + incl - nr;
+      pack_ranges(starts, ends, nc, l, leaf, [&](int r, int a, int c) {
+        const int slot = first_slot + r;
+        if (slot < n) waves[slot] = make_int2(a, c);  // at most n ranges exist
+      });
     }
   }
+}
+
+// ---- 6a. the top levels from per-cell searches ---------------------------------------
+// Levels 0..kTopLevels are built without the level loop: a cell of level l is a
+// Morton prefix of 3l bits, so its body range in the sorted codes is found by a
+// lower-bound search for its smallest code, independently for every cell.
+// A cell is a node when it is the root, or it holds bodies and its parent holds
+// more than leaf_size; nodes are numbered breadth-first with children in digit
+// order, exactly as the level loop numbers them. The loop then continues from
+// level kTopLevels.
+constexpr int kTopLevels = 4;
+constexpr int kTopCells = (1 + 8 + 64 + 512 + 4096);  // levels 0..4
+constexpr int kTopEntries = kTopCells + kTopLevels + 1;  // plus one end sentinel per level
+
+__host__ __device__ constexpr int top_cell_base(int l) {  // first cell of level l
+  return l == 0 ? 0 : top_cell_base(l - 1) + (1 << (3 * (l - 1)));
+}
+__host__ __device__ constexpr int top_entry_base(int l) {  // first entry of level l
+  return top_cell_base(l) + l;
+}
+
+// top_start[top_entry_base(l) + c] = first sorted body of cell c of level l
+// (c = 8^l gives n).
+__global__ __launch_bounds__(256) void k_top_bounds(const uint32_t* __restrict__ codes, int n,
+                                                    int* __restrict__ top_start) {
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= kTopEntries) return;
+  int l = 0;
+  while (l < kTopLevels && e >= top_entry_base(l + 1)) ++l;
+  const uint32_t c = (uint32_t)(e - top_entry_base(l));
+  if (c == (1u << (3 * l))) {
+    top_start[e] = n;
+    return;
+  }
+  const uint32_t key = c << (30 - 3 * l);
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (codes[mid] < key) lo = mid + 1;
+    else hi = mid;
+  }
+  top_start[e] = lo;
+}
+
+// One workgroup: node links of levels 0..kTopLevels, level offsets, the wave
+// ranges of the nodes above kTopLevels, and the range counter / traversal ticket.
+__global__ __launch_bounds__(1024) void k_top_build(const int* __restrict__ top_start, int n,
+                                                    int leaf, int4* __restrict__ link,
+                                                    int* __restrict__ level_off,
+                                                    int2* __restrict__ waves,
+                                                    int* __restrict__ wctl) {
+  __shared__ int s_idx[kTopCells];   // exclusive scan of existence, then node index
+  __shared__ int s_next[kTopCells];  // rope of each node
+  __shared__ int s_part[1024];
+  __shared__ int s_ranges;
+  const int tid = threadIdx.x;
+  auto cell_count = [&](int l, int c) {
+    const int e = top_entry_base(l) + c;
+    return top_start[e + 1] - top_start[e];
+  };
+  auto exists = [&](int l, int c) {
+    return l == 0 || (cell_count(l, c) > 0 && cell_count(l - 1, c >> 3) > leaf);
+  };
+  // 1. existence flags and their exclusive scan (cells in level order)
+  constexpr int kPer = (kTopCells + 1023) / 1024;
+  int flag[kPer];
+  int sum = 0;
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int g = tid * kPer + q;
+    int f = 0;
+    if (g < kTopCells) {
+      int l = 0;
+      while (l < kTopLevels && g >= top_cell_base(l + 1)) ++l;
+      f = exists(l, g - top_cell_base(l)) ? 1 : 0;
+    }
+    flag[q] = f;
+    sum += f;
+  }
+  s_part[tid] = sum;
+  if (tid == 0) s_ranges = 0;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan of the per-thread sums
+    const int v = tid >= off ? s_part[tid - off] : 0;
+    __syncthreads();
+    s_part[tid] += v;
+    __syncthreads();
+  }
+  int run = s_part[tid] - sum;
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const int g = tid * kPer + q;
+    if (g < kTopCells) s_idx[g] = flag[q] ? run : -1;
+#pragma unroll
+    for (int l = 0; l <= kTopLevels; ++l)  // level_off[l] = nodes in the levels above l
+      if (g == top_cell_base(l)) level_off[l] = run;
+    run += flag[q];
+  }
+  if (tid == 1023) level_off[kTopLevels + 1] = s_part[1023];
+  __syncthreads();
+  if (tid == 0) {
+    wctl[1] = 0;  // traversal ticket
+    if (n <= kWaveBodies) {
+      waves[0] = make_int2(0, n);
+      s_ranges = 1;
+    }
+  }
+  // 2. links, level by level (a node's rope falls back to its parent's)
+  for (int l = 0; l <= kTopLevels; ++l) {
+    const int base = top_cell_base(l), cells = 1 << (3 * l);
+    for (int c = tid; c < cells; c += 1024) {
+      const int i = s_idx[base + c];
+      if (i < 0) continue;
+      const int cnt = cell_count(l, c);
+      const int start = top_start[top_entry_base(l) + c];
+      int nxt = l == 0 ? -1 : s_next[top_cell_base(l - 1) + (c >> 3)];
+      for (int sib = c + 1; sib < (c | 7) + 1 && l > 0; ++sib)
+        if (s_idx[base + sib] >= 0) {
+          nxt = s_idx[base + sib];
+          break;
+        }
+      s_next[base + c] = nxt;
+      int first = -1;
+      const bool internal = cnt > leaf && l < kMaxLevel;
+      if (internal && l < kTopLevels) {
+        const int cb = top_cell_base(l + 1) + 8 * c;
+        for (int d = 0; d < 8; ++d)
+          if (s_idx[cb + d] >= 0) {
+            first = s_idx[cb + d];
+            break;
+          }
+      }
+      link[i] = make_int4(first, nxt, start, (cnt << 4) | l);
+      // wave ranges of the children of a node above kTopLevels holding more
+      // than kWaveBodies bodies (level kTopLevels is done by the level loop)
+      if (internal && l < kTopLevels && cnt > kWaveBodies) {
+        int starts[8], ends[8], nc = 0;
+        const int eb = top_entry_base(l + 1) + 8 * c;
+        for (int d = 0; d < 8; ++d)
+          if (top_start[eb + d + 1] > top_start[eb + d]) {
+            starts[nc] = top_start[eb + d];
+            ends[nc] = top_start[eb + d + 1];
+            ++nc;
+          }
+        pack_ranges(starts, ends, nc, l, leaf, [&](int, int a, int k) {
+          const int slot = atomicAdd(&s_ranges, 1);
+          if (slot < n) waves[slot] = make_int2(a, k);
+        });
+      }
+    }
+    __syncthreads();
+  }
+  if (tid == 0) wctl[0] = s_ranges;
 }
 
 // ---- 7. centre of mass ------------------------------------------------------------------
@@ -1035,6 +1220,7 @@ int nb_bh_alloc(nb_ctx* c) {
       A((void**)&b.level_off, sizeof(int) * (kMaxLevel + 2)) ||
       A((void**)&b.child_cnt, sizeof(int) * n) || A((void**)&b.child_off, sizeof(int) * n) ||
       A((void**)&b.counters, sizeof(int) * 16) ||
+      A((void**)&b.top_start, sizeof(int) * kTopEntries) ||
       A((void**)&b.waves, sizeof(int2) * n) ||
       A((void**)&b.inter, sizeof(unsigned long long) * 2))
     return nb_fail(NB_E_OOM, "nb_create: device allocation failed (octree)");
@@ -1049,7 +1235,7 @@ void nb_bh_free(nb_ctx* c) {
   void* ps[] = {b.bbox_part, b.geom,     b.keys[0],  b.keys[1],   b.vals[0],    b.vals[1],
                 b.codes,     b.digit_hist, b.pms,    b.prefix,    b.scan_part,  b.node_com,
                 b.node_link, b.level_off, b.child_cnt, b.child_off, b.iscan_part, b.counters, b.waves,
-                b.inter};
+                b.inter,     b.top_start};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   b = nb_bh_state{};
@@ -1118,11 +1304,13 @@ hipError_t nb_bh_build(nb_ctx* c) {
   hipLaunchKernelGGL(k_mscan_partials, dim3(1), dim3(kScanThreads), 0, s, b.scan_part, mblocks);
   hipLaunchKernelGGL(k_mscan_final, dim3(mblocks), dim3(kScanThreads), 0, s, b.pms, n, b.scan_part,
                      b.prefix);
-  // 6. octree, level by level
-  hipLaunchKernelGGL(k_tree_init, dim3(1), dim3(64), 0, s, n, b.node_link, b.level_off, b.waves,
-                     b.counters);
+  // 6. octree: the top levels from per-cell searches, then level by level
   const int leaf = c->cfg.leaf_size;
-  for (int l = 0; l < kMaxLevel; ++l) {
+  hipLaunchKernelGGL(k_top_bounds, dim3((kTopEntries + 255) / 256), dim3(256), 0, s, codes_sorted,
+                     n, b.top_start);
+  hipLaunchKernelGGL(k_top_build, dim3(1), dim3(1024), 0, s, b.top_start, n, leaf, b.node_link,
+                     b.level_off, b.waves, b.counters);
+  for (int l = kTopLevels; l < kMaxLevel; ++l) {
     const int bound = level_bound(n, l);
     const int grid = std::max(1, std::min((bound + 255) / 256, 4096));
     hipLaunchKernelGGL(k_tree_count, dim3(grid), dim3(256), 0, s, codes_sorted, n, b.node_link,

@@ -53,6 +53,7 @@ struct nb_bh_state {
   int* iscan_part = nullptr;    // int scan block partials
   int* counters = nullptr;      // [0] wave ranges, [1] traversal ticket
   int2* waves = nullptr;        // wave ranges (first sorted body, count <= 64)
+  int* top_start = nullptr;     // first sorted body of every cell of the top levels
   int persist_blocks = 0;       // traversal workgroups (persistent)
   unsigned long long* inter = nullptr;  // interaction counter (diagnostic traversal)
 };
