@@ -263,6 +263,48 @@ __global__ __launch_bounds__(kScanThreads) void k_iscan_final(const int* __restr
   }
 }
 
+// Exclusive scan of at most kSingleScan elements by one workgroup (8 consecutive
+// elements per thread), for the small scans of small systems: one launch
+// instead of three. Writes level_off[l + 2] like k_iscan_partials.
+constexpr int kSingleScan = 8192;
+__global__ __launch_bounds__(1024) void k_iscan_single(const int* __restrict__ in,
+                                                       const int* level_off, int l, int n_const,
+                                                       int cap, int* __restrict__ out,
+                                                       int* level_off_mut) {
+  const int n = scan_count(level_off, l, n_const, cap);
+  constexpr int kPer = kSingleScan / 1024;
+  const int base = threadIdx.x * kPer;
+  int v[kPer];
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0;
+    sum += v[k];
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  int incl = sum;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(incl, off);
+    if (lane >= off) incl += t;
+  }
+  __shared__ int wsum[16];
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  int before = 0, total = 0;
+  for (int w = 0; w < 16; ++w) {  // fixed order
+    before += w < wave ? wsum[w] : 0;
+    total += wsum[w];
+  }
+  int run = before + incl - sum;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0 && level_off_mut) level_off_mut[l + 2] = level_off[l + 1] + total;
+}
+
 // FP64 prefix of (m, m x, m y, m z): prefix[k] = sum over sorted bodies < k.
 __device__ __forceinline__ double4 body_moments(float4 p) {
   const double m = p.w;
@@ -1185,6 +1227,11 @@ hipError_t iscan(const int* in, int* out, int* part, const int* level_off, int l
                  int bound, int* level_off_mut, hipStream_t s) {
   const int blocks = (bound + kScanTile - 1) / kScanTile;
   if (blocks < 1) return hipSuccess;
+  if (bound <= kSingleScan) {
+    hipLaunchKernelGGL(k_iscan_single, dim3(1), dim3(1024), 0, s, in, level_off, l, n_const, bound,
+                       out, level_off_mut);
+    return hipGetLastError();
+  }
   hipLaunchKernelGGL(k_iscan_reduce, dim3(blocks), dim3(kScanThreads), 0, s, in, level_off, l,
                      n_const, bound, part);
   hipLaunchKernelGGL(k_iscan_partials, dim3(1), dim3(kScanThreads), 0, s, part, blocks,
