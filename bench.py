@@ -277,14 +277,14 @@ def run_stream(nb, jobs, limit_s: float, **server_kw):
                 break
             running, finished = s.tick()
             if finished:
-                for jid in list(ids):
+                for jid in s.poll():
                     st = s.status(jid)
                     if st["state"] == nb.JOB_DONE:
                         done.append({"n": ids[jid]["n"], "steps": st["steps"],
                                      "latency": st["t_done"] - st["t_submit"]})
                         body_steps += ids[jid]["n"] * st["steps"]
-                        s.forget(jid)
-                        del ids[jid]
+                    s.forget(jid)
+                    del ids[jid]
             elif running == 0 and k < len(jobs):
                 time.sleep(min(0.01, max(0.0, jobs[k]["t"] - now)))
         queued = running = 0

@@ -282,6 +282,9 @@ int nb_server_submit(nb_server* s, const nb_job* job, int64_t* id);
 /* Admit what fits, advance every running job one step, retire finished jobs.
  * running/finished (optional) receive the running count and the jobs retired. */
 int nb_server_tick(nb_server* s, int32_t* running, int32_t* finished);
+/* Ids of the jobs retired (done or failed) since the last poll, oldest first:
+ * up to cap of them are moved into ids, *count receives how many. */
+int nb_server_poll(nb_server* s, int64_t* ids, int32_t cap, int32_t* count);
 int nb_server_status(nb_server* s, int64_t id, nb_job_status* out);
 /* FP64 final state of a finished job (double[3n] each, interleaved). */
 int nb_server_result(nb_server* s, int64_t id, double* pos3, double* vel3);

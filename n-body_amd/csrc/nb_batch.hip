@@ -310,6 +310,7 @@ int check_slot(nb_batch* b, int32_t id) {
 extern "C" {
 
 int nb_batch_create(nb_batch** out, const nb_batch_config* cfg) {
+  NB_API_SHARED;
   if (!out || !cfg) return nb_fail(NB_E_ARG, "nb_batch_create: null argument");
   *out = nullptr;
   if (cfg->max_bodies < 256 || cfg->max_bodies >= (int64_t(1) << 30))
@@ -379,6 +380,7 @@ int nb_batch_create(nb_batch** out, const nb_batch_config* cfg) {
 }
 
 void nb_batch_destroy(nb_batch* b) {
+  NB_API_SHARED;
   if (!b) return;
   (void)hipStreamSynchronize(b->stream);
   free_batch(b);
@@ -386,6 +388,7 @@ void nb_batch_destroy(nb_batch* b) {
 
 int nb_batch_add(nb_batch* b, const double* pos3, const double* vel3, const double* acc3,
                  const float* mass, int64_t n, float eps2, float dt, float G, int32_t* id) {
+  NB_API_SHARED;
   if (!b || !pos3 || !vel3 || !mass || !id) return nb_fail(NB_E_ARG, "nb_batch_add: null argument");
   if (n < 1) return nb_fail(NB_E_ARG, "nb_batch_add: n < 1");
   if (!(eps2 >= 0.f) || !std::isfinite(eps2) || !std::isfinite(dt))
@@ -446,6 +449,7 @@ int nb_batch_add(nb_batch* b, const double* pos3, const double* vel3, const doub
 }
 
 int nb_batch_remove(nb_batch* b, int32_t id) {
+  NB_API_SHARED;
   int rc = check_slot(b, id);
   if (rc) return rc;
   NB_HIP(hipStreamSynchronize(b->stream));
@@ -478,10 +482,14 @@ int nb_batch_remove(nb_batch* b, int32_t id) {
 int nb_batch_step(nb_batch* b, int32_t nsteps) {
   if (!b) return nb_fail(NB_E_ARG, "nb_batch_step: null batch");
   if (nsteps < 0) return nb_fail(NB_E_ARG, "nb_batch_step: nsteps < 0");
-  int rc = rebuild_tables(b);
-  if (rc) return rc;
+  {
+    NB_API_SHARED;  // the rebuild copies and synchronises; released before any capture
+    int rc = rebuild_tables(b);
+    if (rc) return rc;
+  }
   if (nsteps == 0) return NB_OK;
   if (b->timing) {
+    NB_API_SHARED;
     for (int s = 0; s < nsteps; ++s) {
       (void)hipEventRecord(b->ev[0], b->stream);
       NB_HIP(enqueue_step(b));
@@ -502,6 +510,7 @@ int nb_batch_step(nb_batch* b, int32_t nsteps) {
     return NB_OK;
   }
   if (!b->graph_exec) {
+    NB_API_CAPTURE;
     NB_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
     hipError_t e = enqueue_step(b);
     hipGraph_t g = nullptr;
@@ -518,6 +527,7 @@ int nb_batch_step(nb_batch* b, int32_t nsteps) {
 }
 
 int nb_batch_sync(nb_batch* b) {
+  NB_API_SHARED;
   if (!b) return nb_fail(NB_E_ARG, "nb_batch_sync: null batch");
   NB_HIP(hipStreamSynchronize(b->stream));
   return NB_OK;
@@ -534,6 +544,7 @@ static int copy_slices(nb_batch* b, int32_t id, const double* src, double* dst) 
 }
 
 int nb_batch_forces(nb_batch* b, int32_t id, double* force3) {
+  NB_API_SHARED;
   int rc = check_slot(b, id);
   if (rc) return rc;
   if (!force3) return nb_fail(NB_E_ARG, "nb_batch_forces: null output");
@@ -545,6 +556,7 @@ int nb_batch_forces(nb_batch* b, int32_t id, double* force3) {
 }
 
 int nb_batch_download(nb_batch* b, int32_t id, double* pos3, double* vel3, double* acc3) {
+  NB_API_SHARED;
   int rc = check_slot(b, id);
   if (rc) return rc;
   NB_HIP(hipStreamSynchronize(b->stream));
@@ -555,6 +567,7 @@ int nb_batch_download(nb_batch* b, int32_t id, double* pos3, double* vel3, doubl
 }
 
 int nb_batch_energy(nb_batch* b, int32_t id, double* ekin, double* epot) {
+  NB_API_SHARED;
   int rc = check_slot(b, id);
   if (rc) return rc;
   const int n = b->h_n[id], off = b->h_off[id];
@@ -592,6 +605,7 @@ int nb_batch_info(nb_batch* b, int32_t* clusters, int64_t* bodies, int64_t* inte
 }
 
 int nb_batch_set_timing(nb_batch* b, int32_t enable) {
+  NB_API_SHARED;
   if (!b) return nb_fail(NB_E_ARG, "nb_batch_set_timing: null batch");
   NB_HIP(hipStreamSynchronize(b->stream));
   b->timing = enable != 0;
@@ -607,6 +621,7 @@ int nb_batch_get_timing(nb_batch* b, nb_timing* out) {
 }
 
 int nb_batch_tune(nb_batch* b, int32_t reps, nb_tuning* out) {
+  NB_API_SHARED;
   if (!b) return nb_fail(NB_E_ARG, "nb_batch_tune: null batch");
   if (reps < 1) reps = 3;
   NB_HIP(hipStreamSynchronize(b->stream));

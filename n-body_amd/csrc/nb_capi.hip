@@ -24,6 +24,11 @@ int nb_fail(int code, const std::string& msg) {
 
 extern "C" const char* nb_last_error(void) { return g_last_error.c_str(); }
 
+std::shared_mutex& nb_capture_mutex() {
+  static std::shared_mutex m;
+  return m;
+}
+
 namespace {
 
 template <typename T>
@@ -157,6 +162,7 @@ static int accumulate_timing(nb_ctx* c) {
 extern "C" {
 
 int nb_create(nb_ctx** out, const nb_config* cfg) {
+  NB_API_SHARED;
   if (!out || !cfg) return nb_fail(NB_E_ARG, "nb_create: null argument");
   *out = nullptr;
   if (cfg->n < 1 || cfg->n >= (int64_t(1) << 30))
@@ -237,6 +243,7 @@ int nb_create(nb_ctx** out, const nb_config* cfg) {
 }
 
 void nb_destroy(nb_ctx* ctx) {
+  NB_API_SHARED;
   if (!ctx) return;
   (void)hipStreamSynchronize(ctx->stream);
   free_ctx(ctx);
@@ -263,6 +270,7 @@ static int upload_common(nb_ctx* c, const std::vector<double>& p, const std::vec
 
 int nb_upload(nb_ctx* c, const float* pos4, const float* vel4, const float* acc4,
               const float* mass, int64_t n) {
+  NB_API_SHARED;
   if (!c || !pos4 || !vel4 || !mass) return nb_fail(NB_E_ARG, "nb_upload: null argument");
   if (n != c->n) return nb_fail(NB_E_ARG, "nb_upload: n differs from the context's n");
   std::vector<double> p(3 * n), v(3 * n), a(3 * n, 0.0);
@@ -277,6 +285,7 @@ int nb_upload(nb_ctx* c, const float* pos4, const float* vel4, const float* acc4
 
 int nb_upload_f64(nb_ctx* c, const double* pos3, const double* vel3, const double* acc3,
                   const float* mass, int64_t n) {
+  NB_API_SHARED;
   if (!c || !pos3 || !vel3 || !mass) return nb_fail(NB_E_ARG, "nb_upload_f64: null argument");
   if (n != c->n) return nb_fail(NB_E_ARG, "nb_upload_f64: n differs from the context's n");
   std::vector<double> p(3 * n), v(3 * n), a(3 * n, 0.0);
@@ -296,6 +305,7 @@ int nb_step(nb_ctx* c, int32_t nsteps) {
   if (nsteps == 0) return NB_OK;
   if (c->timing) {
     // Timed steps run eagerly with HIP events around each phase.
+    NB_API_SHARED;
     for (int s = 0; s < nsteps; ++s) {
       NB_HIP(nb_enqueue_step(c));
       int rc = accumulate_timing(c);
@@ -311,6 +321,7 @@ int nb_step(nb_ctx* c, int32_t nsteps) {
     for (int s = 0; s < nsteps; ++s) NB_HIP(nb_enqueue_step(c));
   } else {
     if (!c->graph_exec) {
+      NB_API_CAPTURE;
       NB_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
       hipError_t e = nb_enqueue_step(c);
       hipGraph_t g = nullptr;
@@ -330,6 +341,7 @@ int nb_step(nb_ctx* c, int32_t nsteps) {
 }
 
 int nb_sync(nb_ctx* c) {
+  NB_API_SHARED;
   if (!c) return nb_fail(NB_E_ARG, "nb_sync: null context");
   NB_HIP(hipStreamSynchronize(c->stream));
   return check_flags(c);
@@ -349,6 +361,7 @@ static int force_pass(nb_ctx* c) {
 }
 
 int nb_forces(nb_ctx* c, float* force4) {
+  NB_API_SHARED;
   if (!c || !force4) return nb_fail(NB_E_ARG, "nb_forces: null argument");
   int rc = force_pass(c);
   if (rc) return rc;
@@ -366,6 +379,7 @@ int nb_forces(nb_ctx* c, float* force4) {
 }
 
 int nb_forces_f64(nb_ctx* c, double* force3) {
+  NB_API_SHARED;
   if (!c || !force3) return nb_fail(NB_E_ARG, "nb_forces_f64: null argument");
   int rc = force_pass(c);
   if (rc) return rc;
@@ -395,6 +409,7 @@ static int download_soa(nb_ctx* c, std::vector<double>& p, std::vector<double>& 
 }
 
 int nb_download(nb_ctx* c, float* pos4, float* vel4, float* acc4) {
+  NB_API_SHARED;
   if (!c) return nb_fail(NB_E_ARG, "nb_download: null context");
   std::vector<double> p, v, a;
   const int rc = download_soa(c, p, v, a);
@@ -409,6 +424,7 @@ int nb_download(nb_ctx* c, float* pos4, float* vel4, float* acc4) {
 }
 
 int nb_download_f64(nb_ctx* c, double* pos3, double* vel3, double* acc3) {
+  NB_API_SHARED;
   if (!c) return nb_fail(NB_E_ARG, "nb_download_f64: null context");
   std::vector<double> p, v, a;
   const int rc = download_soa(c, p, v, a);
@@ -423,6 +439,7 @@ int nb_download_f64(nb_ctx* c, double* pos3, double* vel3, double* acc3) {
 }
 
 int nb_energy(nb_ctx* c, double* ekin, double* epot, double p[3]) {
+  NB_API_SHARED;
   if (!c) return nb_fail(NB_E_ARG, "nb_energy: null context");
   if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_energy: no particle system uploaded");
   const int kin_n = 4 * c->red_blocks;
@@ -452,6 +469,7 @@ int nb_energy(nb_ctx* c, double* ekin, double* epot, double p[3]) {
 static const char kSnapMagic[8] = {'N', 'B', 'S', 'N', 'A', 'P', '0', '1'};
 
 int nb_save_snapshot(nb_ctx* c, const char* path) {
+  NB_API_SHARED;
   if (!c || !path) return nb_fail(NB_E_ARG, "nb_save_snapshot: null argument");
   if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_save_snapshot: nothing uploaded");
   std::vector<double> p, v, a;
@@ -476,6 +494,7 @@ int nb_save_snapshot(nb_ctx* c, const char* path) {
 }
 
 int nb_load_snapshot(nb_ctx* c, const char* path) {
+  NB_API_SHARED;
   if (!c || !path) return nb_fail(NB_E_ARG, "nb_load_snapshot: null argument");
   FILE* f = std::fopen(path, "rb");
   if (!f) return nb_fail(NB_E_ARG, std::string("nb_load_snapshot: cannot open ") + path);
@@ -510,6 +529,7 @@ static int ensure_tmp(nb_ctx* c) {
 }
 
 int nb_reorder(nb_ctx* c) {
+  NB_API_SHARED;
   if (!c) return nb_fail(NB_E_ARG, "nb_reorder: null context");
   if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_reorder: nothing uploaded");
   int rc = nb_sort_alloc(c);
@@ -533,6 +553,7 @@ int nb_reorder(nb_ctx* c) {
 }
 
 int nb_render_view(nb_ctx* c, float* xyzs) {
+  NB_API_SHARED;
   if (!c || !xyzs) return nb_fail(NB_E_ARG, "nb_render_view: null argument");
   if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_render_view: nothing uploaded");
   int rc = ensure_tmp(c);
@@ -557,6 +578,7 @@ int nb_get_order(nb_ctx* c, int32_t* order) {
 }
 
 int nb_set_timing(nb_ctx* c, int32_t enable) {
+  NB_API_SHARED;
   if (!c) return nb_fail(NB_E_ARG, "nb_set_timing: null context");
   NB_HIP(hipStreamSynchronize(c->stream));
   c->timing = enable != 0;
@@ -571,6 +593,7 @@ int nb_get_timing(nb_ctx* c, nb_timing* out) {
 }
 
 int nb_count_interactions(nb_ctx* c, int64_t* interactions) {
+  NB_API_SHARED;
   if (!c || !interactions) return nb_fail(NB_E_ARG, "nb_count_interactions: null argument");
   if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_count_interactions: nothing uploaded");
   if (c->cfg.solver != NB_SOLVER_BH) {

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <mutex>
+#include <shared_mutex>
 #include <string>
 #include <vector>
 
@@ -18,6 +20,14 @@ int nb_fail(int code, const std::string& msg);
     if (e_ != hipSuccess)                                                          \
       return nb_fail(NB_E_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+// Stream capture and the runtime calls that may not overlap it. A graph
+// capture (the first nb_step / nb_batch_step of a context) holds this lock
+// exclusively; entry points that allocate, free, copy or synchronise hold it
+// shared. Different contexts may therefore be used from different threads.
+std::shared_mutex& nb_capture_mutex();
+#define NB_API_SHARED std::shared_lock<std::shared_mutex> nb_api_guard_(nb_capture_mutex())
+#define NB_API_CAPTURE std::unique_lock<std::shared_mutex> nb_capture_guard_(nb_capture_mutex())
 
 // Padding granule of the FP32 body array: the direct kernel's i-blocks hold
 // 64 lanes x up to 4 bodies, and its j-range is split over up to 4 waves in

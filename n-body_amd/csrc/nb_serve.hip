@@ -10,15 +10,19 @@
 //              initial conditions. A job estimated above the whole tick budget is
 //              selected only onto an idle device. Later, smaller jobs may pass a
 //              job that does not fit yet (bounded by the look-ahead);
-//   2. start   selected jobs whose initial conditions are ready are uploaded:
-//              direct-sum jobs become clusters of the shared nb_batch pool,
-//              Barnes-Hut jobs get their own nb_ctx (own stream);
+//   2. start   selected jobs whose initial conditions are ready start running:
+//              direct-sum jobs become clusters of the shared nb_batch pool;
+//              Barnes-Hut jobs have their own nb_ctx (own stream), which the
+//              worker that made their initial conditions already created and
+//              uploaded;
 //   3. step    one nb_batch_step for every direct-sum job at once (one persistent
 //              force launch) and one nb_step per Barnes-Hut job, all enqueued
 //              before any wait, so Barnes-Hut kernels overlap with each other and
 //              with the batch;
 //   4. retire  jobs that reached their step count copy their FP64 state to the
-//              host and release their reservations.
+//              host and release their reservations; a Barnes-Hut job's copy and
+//              context teardown run on a worker thread, so the loop never waits
+//              on a large transfer while the device could be stepping.
 // Only selected jobs hold host memory for initial conditions, so a long queue
 // costs a few bytes per job.
 #include <algorithm>
@@ -52,6 +56,8 @@ struct Job {
   nb_ctx* ctx = nullptr;
   // initial conditions (made by a worker) and, later, the final state
   std::atomic<int> prepared{0};  // 0 pending, 1 ready, -1 failed
+  std::atomic<int> finished{0};  // Barnes-Hut retirement on a worker: 1 done, -1 failed
+  double t_fin = 0;              // written by that worker before `finished`
   std::vector<double> pos, vel;
   std::vector<float> mass;
 };
@@ -97,11 +103,14 @@ struct nb_server {
   std::deque<int64_t> queue;       // not yet selected, arrival order
   std::vector<int64_t> preparing;  // selected, initial conditions pending or ready
   std::vector<int64_t> running;    // on the device
+  std::vector<int64_t> retiring;   // Barnes-Hut jobs a worker is copying out
+  std::vector<int64_t> retired;    // finished or failed since the last nb_server_poll
   // initial-condition workers
   std::mutex mu;
   std::condition_variable cv;        // workers wait for jobs to prepare
   std::condition_variable ready_cv;  // an idle tick waits for a prepared job
   std::deque<Job*> prep_queue;
+  std::deque<Job*> retire_queue;
   bool stop = false;
   std::vector<std::thread> workers;
 
@@ -109,17 +118,64 @@ struct nb_server {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
 
+  // Barnes-Hut context for a prepared job: created and uploaded here, on the
+  // job's own stream, so the serving loop does not wait for the copy.
+  int make_context(Job& j) {
+    nb_config c{};
+    c.n = j.desc.n;
+    c.eps2 = j.desc.eps2;
+    c.dt = j.desc.dt;
+    c.G = j.desc.G;
+    c.theta = cfg.theta;
+    c.leaf_size = cfg.leaf_size;
+    c.solver = NB_SOLVER_BH;
+    nb_ctx* ctx = nullptr;
+    int rc = nb_create(&ctx, &c);
+    if (rc != NB_OK) return rc;
+    rc = nb_upload_f64(ctx, j.pos.data(), j.vel.data(), nullptr, j.mass.data(), j.desc.n);
+    if (rc != NB_OK) {
+      nb_destroy(ctx);
+      return rc;
+    }
+    j.ctx = ctx;
+    std::vector<double>().swap(j.pos);
+    std::vector<double>().swap(j.vel);
+    return NB_OK;
+  }
+
   void worker() {
     for (;;) {
       Job* j = nullptr;
+      bool out = false;
       {
         std::unique_lock<std::mutex> lk(mu);
-        cv.wait(lk, [&] { return stop || !prep_queue.empty(); });
+        cv.wait(lk, [&] { return stop || !retire_queue.empty() || !prep_queue.empty(); });
         if (stop) return;
-        j = prep_queue.front();
-        prep_queue.pop_front();
+        out = !retire_queue.empty();  // copies out first: they free device memory
+        std::deque<Job*>& q = out ? retire_queue : prep_queue;
+        j = q.front();
+        q.pop_front();
       }
       const int64_t n = j->desc.n;
+      if (out) {
+        int rc = NB_OK;
+        try {
+          j->pos.resize(3 * n);
+          j->vel.resize(3 * n);
+          rc = nb_download_f64(j->ctx, j->pos.data(), j->vel.data(), nullptr);
+        } catch (const std::bad_alloc&) {
+          rc = NB_E_OOM;
+        }
+        nb_destroy(j->ctx);
+        j->ctx = nullptr;
+        j->t_fin = now();
+        {
+          std::lock_guard<std::mutex> lk(mu);
+          j->finished.store(rc == NB_OK ? 1 : -1);
+        }
+        ready_cv.notify_all();
+        continue;
+      }
       int rc = -1;
       try {
         j->pos.resize(3 * n);
@@ -131,6 +187,7 @@ struct nb_server {
                  : nb_ic_plummer(j->desc.seed, n, 50.0, 1e4, 10.0,
                                  j->desc.G == 0.f ? 1.0 : j->desc.G, j->pos.data(),
                                  j->vel.data(), j->mass.data());
+        if (rc == 0 && j->solver == NB_SOLVER_BH) rc = make_context(*j) == NB_OK ? 0 : -1;
       } catch (const std::bad_alloc&) {
         rc = -1;
       }
@@ -142,55 +199,42 @@ struct nb_server {
     }
   }
 
-  void release(Job& j) {
-    bytes_left += j.bytes;
-    pool_left += j.pool_bodies;
+  void release_time(Job& j) {
     load_ms -= j.cost_ms;
     if (load_ms < 1e-9) load_ms = 0;
-    j.bytes = j.pool_bodies = 0;
     j.cost_ms = 0;
+  }
+  void release(Job& j) {
+    release_time(j);
+    bytes_left += j.bytes;
+    pool_left += j.pool_bodies;
+    j.bytes = j.pool_bodies = 0;
   }
 };
 
 namespace {
 
-// Upload a prepared job. NB_E_BUDGET: the pool has no free segment of that
-// size yet (fragmentation); the job stays prepared and is retried next tick.
+// Start a prepared job. A direct-sum job is uploaded into the shared pool here;
+// NB_E_BUDGET means the pool has no free segment of that size yet
+// (fragmentation), and the job stays prepared and is retried next tick. A
+// Barnes-Hut job's context was created and uploaded by its worker.
 int start(nb_server* s, Job& j) {
-  const int64_t n = j.desc.n;
   if (j.solver == NB_SOLVER_DIRECT) {
     int32_t id = -1;
-    const int rc = nb_batch_add(s->batch, j.pos.data(), j.vel.data(), nullptr, j.mass.data(), n,
-                                j.desc.eps2, j.desc.dt, j.desc.G, &id);
+    const int rc = nb_batch_add(s->batch, j.pos.data(), j.vel.data(), nullptr, j.mass.data(),
+                                j.desc.n, j.desc.eps2, j.desc.dt, j.desc.G, &id);
     if (rc != NB_OK) return rc;
     j.slot = id;
-  } else {
-    nb_config c{};
-    c.n = n;
-    c.eps2 = j.desc.eps2;
-    c.dt = j.desc.dt;
-    c.G = j.desc.G;
-    c.theta = s->cfg.theta;
-    c.leaf_size = s->cfg.leaf_size;
-    c.solver = NB_SOLVER_BH;
-    int rc = nb_create(&j.ctx, &c);
-    if (rc != NB_OK) return rc == NB_E_OOM ? NB_E_BUDGET : rc;
-    rc = nb_upload_f64(j.ctx, j.pos.data(), j.vel.data(), nullptr, j.mass.data(), n);
-    if (rc != NB_OK) {
-      nb_destroy(j.ctx);
-      j.ctx = nullptr;
-      return rc;
-    }
+    std::vector<double>().swap(j.pos);
+    std::vector<double>().swap(j.vel);
   }
-  // the host copy of the initial conditions is no longer needed
-  std::vector<double>().swap(j.pos);
-  std::vector<double>().swap(j.vel);
-  return NB_OK;
+  return j.ctx || j.solver == NB_SOLVER_DIRECT ? NB_OK : NB_E_HIP;
 }
 
-int retire(nb_server* s, Job& j) {
+// Finish a job on the loop's thread (direct-sum jobs, failed jobs, and
+// Barnes-Hut jobs whose worker has copied them out: `rc` is its result).
+int retire(nb_server* s, Job& j, int rc = NB_OK) {
   const int64_t n = j.desc.n;
-  int rc = NB_OK;
   if (j.solver == NB_SOLVER_DIRECT) {
     if (j.slot >= 0) {
       j.pos.resize(3 * n);
@@ -200,10 +244,7 @@ int retire(nb_server* s, Job& j) {
       if (rc == NB_OK) rc = rr;
       j.slot = -1;
     }
-  } else if (j.ctx) {
-    j.pos.resize(3 * n);
-    j.vel.resize(3 * n);
-    rc = nb_download_f64(j.ctx, j.pos.data(), j.vel.data(), nullptr);
+  } else if (j.ctx) {  // a failed Barnes-Hut job: no copy
     nb_destroy(j.ctx);
     j.ctx = nullptr;
   }
@@ -214,7 +255,7 @@ int retire(nb_server* s, Job& j) {
     std::vector<double>().swap(j.pos);
     std::vector<double>().swap(j.vel);
   }
-  j.t_done = s->now();
+  j.t_done = j.t_fin > 0 ? j.t_fin : s->now();
   return rc;
 }
 
@@ -313,6 +354,22 @@ int nb_server_submit(nb_server* s, const nb_job* job, int64_t* id) {
 int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
   if (!s) return nb_fail(NB_E_ARG, "nb_server_tick: null server");
   int finished = 0;
+  // 0. Barnes-Hut jobs whose worker has copied them out are done
+  auto collect = [&] {
+    for (auto it = s->retiring.begin(); it != s->retiring.end();) {
+      Job& j = *s->jobs.at(*it);
+      const int f = j.finished.load();
+      if (f == 0) {
+        ++it;
+        continue;
+      }
+      retire(s, j, f > 0 ? NB_OK : NB_E_HIP);
+      s->retired.push_back(*it);
+      ++finished;
+      it = s->retiring.erase(it);
+    }
+  };
+  collect();
   // 1. select (arrival order, bounded look-ahead)
   int scanned = 0;
   for (auto it = s->queue.begin(); it != s->queue.end() && scanned < s->cfg.lookahead;) {
@@ -343,15 +400,20 @@ int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
     s->cv.notify_one();
     it = s->queue.erase(it);
   }
-  // Idle with work being prepared: wait (up to 50 ms) for the first prepared
-  // job instead of returning an empty tick.
-  if (s->running.empty() && !s->preparing.empty()) {
-    std::unique_lock<std::mutex> lk(s->mu);
-    s->ready_cv.wait_for(lk, std::chrono::milliseconds(50), [&] {
-      for (int64_t id : s->preparing)
-        if (s->jobs.at(id)->prepared.load() != 0) return true;
-      return false;
-    });
+  // Idle with work on the workers: wait (up to 50 ms) for the first prepared
+  // or copied-out job instead of returning an empty tick.
+  if (s->running.empty() && (!s->preparing.empty() || !s->retiring.empty())) {
+    {
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->ready_cv.wait_for(lk, std::chrono::milliseconds(50), [&] {
+        for (int64_t id : s->preparing)
+          if (s->jobs.at(id)->prepared.load() != 0) return true;
+        for (int64_t id : s->retiring)
+          if (s->jobs.at(id)->finished.load() != 0) return true;
+        return false;
+      });
+    }
+    collect();
   }
   // 2. start the prepared jobs (selection order)
   for (auto it = s->preparing.begin(); it != s->preparing.end();) {
@@ -369,6 +431,7 @@ int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
     if (rc != NB_OK) {
       j.state = NB_JOB_FAILED;
       retire(s, j);
+      s->retired.push_back(*it);
       ++finished;
     } else {
       j.state = NB_JOB_RUNNING;
@@ -409,12 +472,23 @@ int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
     const int rc = nb_batch_sync(s->batch);
     if (rc != NB_OK) return rc;
   }
-  // 4. retire
+  // 4. retire: direct-sum and failed jobs here, Barnes-Hut jobs on a worker
   for (auto it = s->running.begin(); it != s->running.end();) {
     Job& j = *s->jobs.at(*it);
     if (j.steps_done >= j.desc.steps || j.state == NB_JOB_FAILED) {
-      retire(s, j);
-      ++finished;
+      if (j.solver == NB_SOLVER_BH && j.state != NB_JOB_FAILED) {
+        s->release_time(j);  // off the device's schedule; its memory until the copy ends
+        s->retiring.push_back(*it);
+        {
+          std::lock_guard<std::mutex> lk(s->mu);
+          s->retire_queue.push_back(&j);
+        }
+        s->cv.notify_one();
+      } else {
+        retire(s, j);
+        s->retired.push_back(*it);
+        ++finished;
+      }
       it = s->running.erase(it);
     } else {
       ++it;
@@ -422,6 +496,15 @@ int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
   }
   if (running_out) *running_out = (int32_t)s->running.size();
   if (finished_out) *finished_out = finished;
+  return NB_OK;
+}
+
+int nb_server_poll(nb_server* s, int64_t* ids, int32_t cap, int32_t* count) {
+  if (!s || !count || (cap > 0 && !ids)) return nb_fail(NB_E_ARG, "nb_server_poll: bad argument");
+  const int32_t k = (int32_t)std::min<size_t>(cap > 0 ? (size_t)cap : 0, s->retired.size());
+  std::copy(s->retired.begin(), s->retired.begin() + k, ids);
+  s->retired.erase(s->retired.begin(), s->retired.begin() + k);
+  *count = k;
   return NB_OK;
 }
 

@@ -215,6 +215,7 @@ _SIGS = {
     "nb_server_destroy": (None, [_P]),
     "nb_server_submit": (C.c_int, [_P, C.POINTER(Job), C.POINTER(C.c_int64)]),
     "nb_server_tick": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    "nb_server_poll": (C.c_int, [_P, _P, C.c_int32, C.POINTER(C.c_int32)]),
     "nb_server_status": (C.c_int, [_P, C.c_int64, C.POINTER(JobStatus)]),
     "nb_server_result": (C.c_int, [_P, C.c_int64, _P, _P]),
     "nb_server_forget": (C.c_int, [_P, C.c_int64]),
@@ -646,6 +647,15 @@ class Server:
         f = C.c_int32()
         _check(self._lib.nb_server_tick(self._s, C.byref(r), C.byref(f)))
         return r.value, f.value
+
+    def poll(self, cap: int = 4096) -> list:
+        """Ids of the jobs retired since the last poll (nb_server_poll)."""
+        out, buf, cnt = [], np.empty(cap, np.int64), C.c_int32()
+        while True:
+            _check(self._lib.nb_server_poll(self._s, _ptr(buf), cap, C.byref(cnt)))
+            out.extend(int(x) for x in buf[:cnt.value])
+            if cnt.value < cap:
+                return out
 
     def status(self, jid: int) -> dict:
         st = JobStatus()
