@@ -104,3 +104,17 @@ def test_job_that_never_fits_is_refused(nb):
     with nb.Server(direct_pool=1 << 12, bh_threshold=8192) as s:
         with pytest.raises(nb.NBodyError):
             s.submit(5000, 2, 0, 1)  # 5120 padded bodies > the 4096-body pool
+
+
+def test_poll_returns_every_retired_job_once(nb):
+    jobs = [(2000, 3, 0, 31), (12000, 2, 1, 32), (500, 1, 0, 33), (9000, 0, 1, 34)]
+    with nb.Server(bh_threshold=8192, direct_pool=1 << 16) as s:
+        ids = [s.submit(*j) for j in jobs]
+        seen = []
+        t0 = time.monotonic()
+        while time.monotonic() - t0 < 60 and len(seen) < len(ids):
+            s.tick()
+            seen += s.poll()
+        assert sorted(seen) == sorted(ids) and len(set(seen)) == len(seen)
+        assert s.poll() == []
+        assert all(s.status(i)["state"] == nb.JOB_DONE for i in ids)
