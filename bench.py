@@ -146,7 +146,30 @@ def cpu_baseline_bh(pos, mass, budget_s: float):
     }
 
 
-def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0):
+def force_errors(pos, mass, force, targets):
+    """Relative force error on `targets` against an independent FP64 direct sum
+    (numpy, G = 1) over all sources, from the FP32 positions the force pass sees.
+    Outside every timed region."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    p = pos.astype(np.float32).astype(np.float64)
+    mm = mass.astype(np.float64)
+
+    def one(t):
+        d = p - p[t]
+        r2 = np.einsum("ij,ij->i", d, d) + EPS2
+        w = mm / (r2 * np.sqrt(r2))
+        w[t] = 0.0
+        ref = w @ d
+        return np.linalg.norm(force[t] - ref) / np.linalg.norm(ref)
+
+    with ThreadPoolExecutor(min(16, host_threads())) as ex:
+        e = np.array(list(ex.map(one, [int(t) for t in targets])))
+    return {"targets": len(targets), "p50": float(np.median(e)), "p99": float(np.percentile(e, 99)),
+            "max": float(e.max())}
+
+
+def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0, accuracy: bool = True):
     n = 1 << 22
     t0 = time.perf_counter()
     pos, vel, m = nb.ic_discs(2024, n)
@@ -162,6 +185,14 @@ def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0):
         sim.sync()
         wall = time.perf_counter() - t0
         t = sim.timing()
+        acc = None
+        if accuracy:  # forces at the final state, 64 seeded targets
+            sim.set_timing(False)
+            pf, _, _ = sim.download_f64()
+            f = sim.forces_f64()
+            tg = np.random.default_rng(7).choice(n, 64, replace=False)
+            acc = force_errors(pf, m, f, tg)
+            log(f"bh: force error vs FP64 p50 {acc['p50']:.2e} p99 {acc['p99']:.2e}")
     ms = 1e3 * wall / steps
     cpu = None
     if cpu_budget > 0:
@@ -177,6 +208,7 @@ def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0):
         "traverse_ms": t["force_ms"] / max(1, t["force_calls"]),
         "interactions_first_step": inter,
         "interactions_per_s": inter / (t["force_ms"] / max(1, t["force_calls"]) / 1e3),
+        "force_error_vs_fp64": acc,
         "cpu_baseline": cpu,
     }
 
@@ -415,6 +447,8 @@ def main():
                     help="also run the serving stream for this many seconds per mode")
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (built-in shapes)")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the FP64 force-error and energy-drift report (outside the timed region)")
     ap.add_argument("--only", choices=["direct", "bh", "batch"], default=None,
                     help="run one workload alone (for profiling); prints its sub-object")
     ap.add_argument("--copy-probe", type=float, default=0.0,
@@ -433,7 +467,7 @@ def main():
 
     log(nb.device_info())
     if args.only == "bh":
-        print(json.dumps(bench_bh(nb, args.bh_steps, 2)), flush=True)
+        print(json.dumps(bench_bh(nb, args.bh_steps, 2, 0.0, not args.no_accuracy)), flush=True)
         return
     if args.only == "batch":
         print(json.dumps(bench_batch(nb, args.batch_steps, 2, args.batch_energy, not args.no_tune)),
@@ -450,9 +484,11 @@ def main():
             tuning["bh_4m"] = nb.autotune(1 << 22, nb.TUNE_BH, 2)
         log(f"autotune ({time.perf_counter() - t0:.1f} s): {tuning}")
     pos, vel, m = nb.ic_plummer(2018, n, 50.0, 1e4)
+    accuracy = None
     with nb.Simulation(n, EPS2, DT, bodies_per_lane=args.bodies_per_lane,
                        j_split=args.j_split) as sim:
         sim.upload(pos, vel, m)
+        e0 = sum(sim.energy()[:2]) if not args.no_accuracy else None
         sim.step(args.warmup)
         sim.sync()
         sim.set_timing(True)
@@ -461,6 +497,18 @@ def main():
         sim.sync()
         wall = time.perf_counter() - t0
         t = sim.timing()
+        if not args.no_accuracy:  # outside the timed region
+            sim.set_timing(False)
+            e1 = sum(sim.energy()[:2])
+            pf, _, _ = sim.download_f64()
+            f = sim.forces_f64()
+            tg = np.random.default_rng(5).choice(n, 512, replace=False)
+            accuracy = {"force_error_vs_fp64": force_errors(pf, m, f, tg),
+                        "energy_drift": abs(e1 - e0) / abs(e0),
+                        "energy_steps": args.warmup + args.steps}
+            log(f"direct: force error vs FP64 p50 {accuracy['force_error_vs_fp64']['p50']:.2e} "
+                f"p99 {accuracy['force_error_vs_fp64']['p99']:.2e}; energy drift "
+                f"{accuracy['energy_drift']:.2e} over {accuracy['energy_steps']} steps")
     inter = float(n) * (n - 1)
     value = inter * args.steps / wall
     force_s = t["force_ms"] / t["force_calls"] / 1e3
@@ -490,6 +538,7 @@ def main():
             "pct_fp32_peak": 100.0 * achieved / FP32_PEAK_TFLOPS,
             "shape": nb.get_tuning(n),
             "autotune": tuning,
+            "accuracy": accuracy,
         },
         "roofline": {
             "bound": "valu",
@@ -503,7 +552,8 @@ def main():
     }
     if not args.no_bh:
         log("bh: N=2^22 two-disc collision")
-        result["bh_4m"] = bench_bh(nb, args.bh_steps, 2, 0.0 if args.no_cpu else args.cpu_budget)
+        result["bh_4m"] = bench_bh(nb, args.bh_steps, 2, 0.0 if args.no_cpu else args.cpu_budget,
+                                   not args.no_accuracy)
         log(f"bh: {result['bh_4m']['ms_per_step']:.2f} ms/step")
     if not args.no_batch:
         log("batch: C4 ensemble")
