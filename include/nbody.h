@@ -23,7 +23,9 @@
  *     (Octree.cpp:15,320; ParticleSolverBHutGPU.h:19).
  *
  * Threading: a context is used by one host thread at a time; distinct
- * contexts may be used concurrently. nb_step is asynchronous on the
+ * contexts may be used concurrently (the library keeps a context's one-time
+ * step-graph capture from overlapping other threads' allocations and copies).
+ * nb_step is asynchronous on the
  * context's stream; nb_download*, nb_forces, nb_energy and nb_sync wait.
  * Every function returns 0 on success or a negative NB_E_* code; the
  * calling thread's nb_last_error() then describes the failure.
@@ -236,7 +238,9 @@ int nb_batch_get_timing(nb_batch* b, nb_timing* out);
  * (n < bh_threshold) share one nb_batch launch; Barnes-Hut jobs run their own
  * contexts on their own streams, so their kernels overlap on the device.
  * Finished jobs release their memory and keep their FP64 final state on the
- * host until fetched. A job's result is the same alone or in any mix. */
+ * host until fetched. A job's result is the same alone or in any mix.
+ * The reference has no counterpart (one interactive simulation per process,
+ * main.cpp:73-153); this is the engine's serving extension. */
 typedef struct nb_server nb_server;
 typedef struct nb_server_config {
   int64_t hbm_budget;    /* device bytes jobs may hold; 0 = 60% of free memory */

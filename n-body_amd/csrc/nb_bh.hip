@@ -314,14 +314,17 @@ __device__ __forceinline__ double4 add4(double4 a, double4 b) {
   return make_double4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w);
 }
 
+// Tiles of kScanTile = 8 rows x 256 bodies: thread t holds the bodies t, 256 + t,
+// ..., so every wave load and store is one contiguous 1 KB (float4) or 2 KB
+// (double4) run.
 __global__ __launch_bounds__(kScanThreads) void k_mscan_reduce(const float4* __restrict__ pms,
                                                                int n, double4* __restrict__ part) {
-  constexpr int kPer = kScanTile / kScanThreads;
+  constexpr int kRows = kScanTile / kScanThreads;
   const int base = blockIdx.x * kScanTile;
   double4 s = make_double4(0, 0, 0, 0);
 #pragma unroll
-  for (int k = 0; k < kPer; ++k) {
-    const int e = base + threadIdx.x * kPer + k;
+  for (int r = 0; r < kRows; ++r) {
+    const int e = base + r * kScanThreads + threadIdx.x;
     if (e < n) s = add4(s, body_moments(pms[e]));
   }
   __shared__ double4 sh[kScanThreads];
@@ -362,36 +365,48 @@ __global__ __launch_bounds__(kScanThreads) void k_mscan_partials(double4* __rest
   if (threadIdx.x == kScanThreads - 1) part[nparts] = sh[kScanThreads - 1];
 }
 
+__device__ __forceinline__ double4 shfl_up4(double4 v, int off) {
+  return make_double4(__shfl_up(v.x, off), __shfl_up(v.y, off), __shfl_up(v.z, off),
+                      __shfl_up(v.w, off));
+}
+
+// Exclusive prefix of each tile, row by row: a row of 256 bodies is scanned by
+// wave shuffles and a fixed-order combine of the 4 wave totals; rows follow
+// each other in order. Deterministic (no atomics, fixed association).
 __global__ __launch_bounds__(kScanThreads) void k_mscan_final(const float4* __restrict__ pms,
                                                               int n,
                                                               const double4* __restrict__ part,
                                                               double4* __restrict__ prefix) {
-  constexpr int kPer = kScanTile / kScanThreads;
+  constexpr int kRows = kScanTile / kScanThreads;
+  constexpr int kWaves = kScanThreads / 64;
   const int base = blockIdx.x * kScanTile;
-  double4 v[kPer];
-  double4 s = make_double4(0, 0, 0, 0);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ double4 wtot[kRows][kWaves];
+  double4 incl[kRows];
 #pragma unroll
-  for (int k = 0; k < kPer; ++k) {
-    const int e = base + threadIdx.x * kPer + k;
-    v[k] = e < n ? body_moments(pms[e]) : make_double4(0, 0, 0, 0);
-    s = add4(s, v[k]);
+  for (int r = 0; r < kRows; ++r) {
+    const int e = base + r * kScanThreads + threadIdx.x;
+    double4 x = e < n ? body_moments(pms[e]) : make_double4(0, 0, 0, 0);
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const double4 t = shfl_up4(x, off);
+      if (lane >= off) x = add4(x, t);
+    }
+    if (lane == 63) wtot[r][wave] = x;
+    const double4 below = shfl_up4(x, 1);  // inclusive value of the lane below
+    incl[r] = lane > 0 ? below : make_double4(0, 0, 0, 0);
   }
-  __shared__ double4 sh[kScanThreads];
-  sh[threadIdx.x] = s;
   __syncthreads();
-  for (int d = 1; d < kScanThreads; d <<= 1) {
-    const double4 t = threadIdx.x >= d ? sh[threadIdx.x - d] : make_double4(0, 0, 0, 0);
-    __syncthreads();
-    sh[threadIdx.x] = add4(sh[threadIdx.x], t);
-    __syncthreads();
-  }
-  // exclusive start of this thread: block offset + totals of the lower threads
-  double4 run = threadIdx.x > 0 ? add4(part[blockIdx.x], sh[threadIdx.x - 1]) : part[blockIdx.x];
+  double4 run = part[blockIdx.x];  // exclusive start of this tile
 #pragma unroll
-  for (int k = 0; k < kPer; ++k) {
-    const int e = base + threadIdx.x * kPer + k;
-    if (e < n) prefix[e] = run;
-    run = add4(run, v[k]);
+  for (int r = 0; r < kRows; ++r) {
+    double4 before = run;  // tile start + earlier rows + earlier waves of this row
+    for (int w = 0; w < kWaves; ++w) {
+      if (w < wave) before = add4(before, wtot[r][w]);
+      run = add4(run, wtot[r][w]);
+    }
+    const int e = base + r * kScanThreads + threadIdx.x;
+    if (e < n) prefix[e] = add4(before, incl[r]);  // incl[r]: exclusive within the wave
   }
   if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) prefix[n] = part[gridDim.x];
 }
@@ -425,63 +440,98 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, uint64_t active) {
   return m;
 }
 
+// Stable scatter of one 8-bit digit pass. The tile is first reordered in LDS by
+// digit (stable ranks from per-wave ballots, waves and rounds in element
+// order), then written out in LDS order: consecutive threads write the runs of
+// equal digits to consecutive global addresses instead of 64 scattered lines
+// per wave store.
 __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     const uint32_t* __restrict__ keys_in, const uint32_t* __restrict__ vals_in, int n, int shift,
     int nblocks, const int* __restrict__ offsets, uint32_t* __restrict__ keys_out,
     uint32_t* __restrict__ vals_out) {
   __shared__ int whist[4][256];
   __shared__ int wrun[4][256];
+  __shared__ int local_start[256];  // first tile slot of each digit
+  __shared__ int global_start[256];  // first output index of each digit for this tile
+  __shared__ uint32_t sk[kSortTile], sv[kSortTile];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   for (int w = 0; w < 4; ++w) whist[w][threadIdx.x] = 0;
   __syncthreads();
 
   // Wave w owns tile elements [w*1024, (w+1)*1024), 64 consecutive per round.
-  const int base = blockIdx.x * kSortTile + wave * (kSortTile / 4);
+  const int tile0 = blockIdx.x * kSortTile;
+  const int base = tile0 + wave * (kSortTile / 4);
   const uint64_t lt = (uint64_t(1) << lane) - 1;
-  uint32_t k[kSortItems], v[kSortItems];
-#pragma unroll
-  for (int r = 0; r < kSortItems; ++r) {
-    const int e = base + r * 64 + lane;
-    k[r] = e < n ? keys_in[e] : 0u;
-    v[r] = e < n ? vals_in[e] : 0u;
-  }
+  // The rounds read their keys again in phase 2 (coalesced, 4 B per key)
+  // instead of holding 16 keys and values per lane: the kernel then needs a few
+  // dozen VGPRs instead of hundreds, and many waves per SIMD hide the latency.
   // phase 1: per-wave digit counts
-#pragma unroll
+#pragma unroll 1
   for (int r = 0; r < kSortItems; ++r) {
     const int e = base + r * 64 + lane;
     const uint64_t active = __ballot(e < n);
-    const uint32_t d = (k[r] >> shift) & 255u;
+    const uint32_t d = ((e < n ? keys_in[e] : 0u) >> shift) & 255u;
     const uint64_t m = match_digit(d, active);
     if (e < n && (m & lt) == 0) whist[wave][d] += __popcll(m);
   }
   __syncthreads();
   {
+    // tile-local digit starts: exclusive scan of the tile's digit counts
     const int d = threadIdx.x;
-    int run = offsets[d * nblocks + blockIdx.x];
+    const int cnt = whist[0][d] + whist[1][d] + whist[2][d] + whist[3][d];
+    int incl = cnt;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const int t = __shfl_up(incl, off);
+      if (lane >= off) incl += t;
+    }
+    __shared__ int wsum[4];
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    int start = incl - cnt;
+    for (int w = 0; w < wave; ++w) start += wsum[w];
+    local_start[d] = start;
+    global_start[d] = offsets[d * nblocks + blockIdx.x];
+    int run = start;
     for (int w = 0; w < 4; ++w) {
       wrun[w][d] = run;
       run += whist[w][d];
     }
   }
   __syncthreads();
-  // phase 2: stable ranks and scatter
-#pragma unroll
+  // phase 2: stable tile-local ranks, reorder in LDS
+#pragma unroll 1
   for (int r = 0; r < kSortItems; ++r) {
     const int e = base + r * 64 + lane;
     const uint64_t active = __ballot(e < n);
-    const uint32_t d = (k[r] >> shift) & 255u;
+    const uint32_t key = e < n ? keys_in[e] : 0u;
+    const uint32_t val = e < n ? vals_in[e] : 0u;
+    const uint32_t d = (key >> shift) & 255u;
     const uint64_t m = match_digit(d, active);
     if (e < n) {
-      const int dst = wrun[wave][d] + __popcll(m & lt);
-      if (dst >= 0 && dst < n) {  // always true for a consistent histogram
-        keys_out[dst] = k[r];
-        vals_out[dst] = v[r];
+      const int slot = wrun[wave][d] + __popcll(m & lt);
+      if (slot >= 0 && slot < kSortTile) {  // always true for a consistent histogram
+        sk[slot] = key;
+        sv[slot] = val;
       }
     }
     __builtin_amdgcn_wave_barrier();
     if (e < n && (m & lt) == 0) wrun[wave][d] += __popcll(m);
     __builtin_amdgcn_wave_barrier();
+  }
+  __syncthreads();
+  // phase 3: write out in tile order
+  const int count = min(kSortTile, n - tile0);
+#pragma unroll 4
+  for (int i = threadIdx.x; i < count; i += kSortThreads) {
+    const uint32_t key = sk[i];
+    const uint32_t d = (key >> shift) & 255u;
+    const int dst = global_start[d] + (i - local_start[d]);
+    if (dst >= 0 && dst < n) {
+      keys_out[dst] = key;
+      vals_out[dst] = sv[i];
+    }
   }
 }
 
