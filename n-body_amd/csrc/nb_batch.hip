@@ -416,28 +416,39 @@ int nb_batch_add(nb_batch* b, const double* pos3, const double* vel3, const doub
     }
   std::vector<float> m(npad, 0.f);
   std::copy(mass, mass + n, m.begin());
-  for (int k = 0; k < 3; ++k) {
-    NB_HIP(hipMemcpyAsync(b->pos + k * cap + off, p.data() + k * n, n * sizeof(double),
-                          hipMemcpyHostToDevice, b->stream));
-    NB_HIP(hipMemcpyAsync(b->vel + k * cap + off, v.data() + k * n, n * sizeof(double),
-                          hipMemcpyHostToDevice, b->stream));
-    NB_HIP(hipMemcpyAsync(b->acc + k * cap + off, a.data() + k * n, n * sizeof(double),
-                          hipMemcpyHostToDevice, b->stream));
-  }
-  NB_HIP(hipMemcpyAsync(b->mass + off, m.data(), npad * sizeof(float), hipMemcpyHostToDevice,
-                        b->stream));
   const int io = (int)off, in = (int)n, inp = (int)npad;
   const double ddt = dt, dG = G == 0.f ? 1.0 : (double)G;
-  NB_HIP(hipMemcpyAsync(b->c_off + slot, &io, sizeof(int), hipMemcpyHostToDevice, b->stream));
-  NB_HIP(hipMemcpyAsync(b->c_n + slot, &in, sizeof(int), hipMemcpyHostToDevice, b->stream));
-  NB_HIP(hipMemcpyAsync(b->c_npad + slot, &inp, sizeof(int), hipMemcpyHostToDevice, b->stream));
-  NB_HIP(hipMemcpyAsync(b->c_eps2 + slot, &eps2, sizeof(float), hipMemcpyHostToDevice, b->stream));
-  NB_HIP(hipMemcpyAsync(b->c_dt + slot, &ddt, sizeof(double), hipMemcpyHostToDevice, b->stream));
-  NB_HIP(hipMemcpyAsync(b->c_G + slot, &dG, sizeof(double), hipMemcpyHostToDevice, b->stream));
-  hipLaunchKernelGGL(k_batch_refresh, dim3((unsigned)(npad / 256)), dim3(256), 0, b->stream, io, in,
-                     inp, b->cap, b->pos, b->mass, b->pm);
-  NB_HIP(hipGetLastError());
-  NB_HIP(hipStreamSynchronize(b->stream));  // host staging buffers die here
+  auto upload = [&]() -> hipError_t {
+    hipError_t e = hipSuccess;
+    auto put = [&](void* dst, const void* src, size_t bytes) {
+      if (e == hipSuccess) e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, b->stream);
+    };
+    for (int k = 0; k < 3; ++k) {
+      put(b->pos + k * cap + off, p.data() + k * n, n * sizeof(double));
+      put(b->vel + k * cap + off, v.data() + k * n, n * sizeof(double));
+      put(b->acc + k * cap + off, a.data() + k * n, n * sizeof(double));
+    }
+    put(b->mass + off, m.data(), npad * sizeof(float));
+    put(b->c_off + slot, &io, sizeof(int));
+    put(b->c_n + slot, &in, sizeof(int));
+    put(b->c_npad + slot, &inp, sizeof(int));
+    put(b->c_eps2 + slot, &eps2, sizeof(float));
+    put(b->c_dt + slot, &ddt, sizeof(double));
+    put(b->c_G + slot, &dG, sizeof(double));
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_batch_refresh, dim3((unsigned)(npad / 256)), dim3(256), 0, b->stream, io,
+                       in, inp, b->cap, b->pos, b->mass, b->pm);
+    e = hipGetLastError();
+    const hipError_t es = hipStreamSynchronize(b->stream);  // host staging buffers die here
+    return e != hipSuccess ? e : es;
+  };
+  const hipError_t ue = upload();
+  if (ue != hipSuccess) {
+    // give the segment back (coalescing happens on the next remove)
+    if (len > npad) b->free_list.erase(off + npad);
+    b->free_list[off] = len;
+    return nb_fail(NB_E_HIP, std::string("nb_batch_add: upload failed: ") + hipGetErrorString(ue));
+  }
   b->used[slot] = 1;
   b->h_off[slot] = io;
   b->h_n[slot] = in;

@@ -447,9 +447,8 @@ int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
     if (j.steps_done >= j.desc.steps) continue;
     if (j.solver == NB_SOLVER_DIRECT) {
       any_direct = true;
-    } else {
-      const int rc = nb_step(j.ctx, 1);
-      if (rc != NB_OK) return rc;
+    } else if (nb_step(j.ctx, 1) != NB_OK) {
+      j.state = NB_JOB_FAILED;  // this job alone; it is retired below
     }
   }
   if (any_direct) {
@@ -458,7 +457,7 @@ int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
   }
   for (int64_t id : s->running) {
     Job& j = *s->jobs.at(id);
-    if (j.steps_done >= j.desc.steps) continue;
+    if (j.steps_done >= j.desc.steps || j.state == NB_JOB_FAILED) continue;
     if (j.solver == NB_SOLVER_BH) {
       const int rc = nb_sync(j.ctx);
       if (rc != NB_OK) {
