@@ -1036,15 +1036,22 @@ constexpr uint32_t kCellBit = 0x80000000u;    // list entry refers to a cell, no
 // each lane gathers one source (cell COM or body) into the LDS staging area,
 // then the lanes read the staged sources by broadcast. The bodies of a range are
 // held S = 64 / P times (P = its body count rounded up to a power of two, at
-// least 4; S = 1 above 32 bodies): the lanes of slice s = lane / P take the staged entries
-// s, s + S, s + 2S, ..., so every lane does useful work, and the slices' partial
-// sums are added at the end of the range.
+// least 4; S = 1 above 32 bodies): the lanes of slice s = lane / P take the
+// staged entries s, s + S, s + 2S, ..., so every lane does useful work, and the
+// slices' partial sums are added at the end of the range. Entries are staged
+// slice-major (entry j at (j % S) * (64 / S) + j / S), so a slice reads its
+// entries from consecutive addresses: four reads per step share one base
+// address with immediate offsets, and the trip count is wave-uniform.
 template <bool kCount, bool kGuard>
 __device__ __forceinline__ void list_flush(const uint32_t* list, int lc, float4* stage, int lane,
-                                           int slice, int S, const float4* __restrict__ pms,
+                                           int slice, int log2S, const float4* __restrict__ pms,
                                            const float4* __restrict__ com, float4 me, float eps2,
                                            float& ax, float& ay, float& az,
                                            unsigned long long& cnt) {
+  const int S = 1 << log2S;
+  const int per = 64 >> log2S;  // entries per slice
+  const int put = (lane & (S - 1)) * per + (lane >> log2S);  // slice-major slot of entry `lane`
+  const float4* mine = stage + slice * per;
   for (int e = 0; e < lc; e += 64) {
     const int m = min(64, lc - e);
     float4 q = make_float4(0.f, 0.f, 0.f, 0.f);  // zero-mass pad
@@ -1052,16 +1059,16 @@ __device__ __forceinline__ void list_flush(const uint32_t* list, int lc, float4*
       const uint32_t id = list[e + lane];
       q = (id & kCellBit) ? com[id & ~kCellBit] : pms[id];
     }
-    stage[lane] = q;
+    stage[put] = q;
     __builtin_amdgcn_wave_barrier();
-    // entries slice + S*t for t < T, T = ceil(m / S) rounded up to 4: S*T <= 64
-    // because S divides 64 and S <= 16, so every read stays inside the 64 staged
-    // entries; entries at or past m are zero-mass pads.
-    const int T = ((m + S - 1) / S + 3) & ~3;
+    // T = ceil(m / S) rounded up to 4 entries per slice; S <= 16 divides 64, so
+    // 64 / S is a multiple of 4 and T <= 64 / S: every read stays inside this
+    // slice's block; entries at or past m are zero-mass pads.
+    const int T = __builtin_amdgcn_readfirstlane((((m + S - 1) >> log2S) + 3) & ~3);
     for (int t = 0; t < T; t += 4) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const float4 src = stage[slice + S * (t + u)];  // few distinct addresses: LDS broadcast
+        const float4 src = mine[t + u];  // few distinct addresses: LDS broadcast
         const float dx = src.x - me.x, dy = src.y - me.y, dz = src.z - me.z;
         const float r2 = fmaf(dz, dz, fmaf(dy, dy, fmaf(dx, dx, eps2)));
         const float inv = __builtin_amdgcn_rsqf(r2);
@@ -1095,7 +1102,7 @@ __device__ void traverse_range(const WaveLds& L, int lane, int P, float4 me, int
                                int* __restrict__ flags, float& ax, float& ay, float& az,
                                unsigned long long& cnt) {
   const uint64_t lt = (uint64_t(1) << lane) - 1;
-  const int S = 64 / P, slice = lane / P;
+  const int log2S = 6 - (31 - __clz(P)), slice = lane / P;  // S = 64 / P
   float lx = me.x, ly = me.y, lz = me.z, hx = me.x, hy = me.y, hz = me.z;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
@@ -1110,7 +1117,7 @@ __device__ void traverse_range(const WaveLds& L, int lane, int P, float4 me, int
   uint32_t* list = L.list;
 #define NB_FLUSH()                                                                              \
   do {                                                                                          \
-    list_flush<kCount, kGuard>(list, lc, L.stage, lane, slice, S, pms, com, me, eps2, ax, ay, \
+    list_flush<kCount, kGuard>(list, lc, L.stage, lane, slice, log2S, pms, com, me, eps2, ax, ay, \
                                az, cnt);                                                       \
     lc = 0;                                                                                     \
   } while (0)
