@@ -47,6 +47,38 @@ int nb_ic_discs(uint64_t seed, int64_t n, double scale_length, double scale_heig
  * (ParticleSystemLagrange.cpp:8-50), computed in single precision as there. */
 int nb_ic_lagrange(double world_size, double* pos3, double* vel3, float* mass);
 
+/* The reference's visual initializers, seeded; world_size w is the edge of the
+ * reference's world cube (main.cpp:29 uses 5), and every mass scale follows
+ * the reference's (w^3 / n) rule.
+ *
+ * Cube (-i 1, ParticleSystemCubeInitializer.cpp:8-29): positions uniform in
+ * [0, w]^3, velocity components uniform in [-L, L] with L = sqrt(3) w / 20,
+ * masses (w^3/n) * 1.2 * U(0.8, 1.3). */
+int nb_ic_cube(uint64_t seed, int64_t n, double world_size, double* pos3, double* vel3,
+               float* mass);
+/* Cube surface (-i 6, ParticleSystemCubeSurface.cpp:8-49): a face of [0, w]^3
+ * chosen uniformly, uniform over the face; velocity and mass as the cube. */
+int nb_ic_cube_surface(uint64_t seed, int64_t n, double world_size, double* pos3, double* vel3,
+                       float* mass);
+/* Sphere surface (-i 4, ParticleSystemSphere.cpp:20-133): uniform on the sphere
+ * of radius sqrt(3) w / 3 about (w/2, w/2, w/2), masses (w^3/n) * 0.5 * U(0.7, 1.3),
+ * speeds sqrt(3) w / 15 * U(0.5, 1.5). pattern 0: radially outward, 1: inward,
+ * 2: along axis x r with axis = normalize(0.2, 1, -0.1); -1 draws the pattern from
+ * the seed, as the reference draws it once per system. */
+int nb_ic_sphere_surface(uint64_t seed, int64_t n, double world_size, int pattern, double* pos3,
+                         double* vel3, float* mass);
+/* Disc galaxy (-i 2, the reference's default, ParticleSystemGalaxyInitializer.cpp:
+ * 12-210), centred at (w/2, w/2, w/2) in the z = w/2 plane. style 0 (lines 46-91):
+ * radius U(0, w), angle 2 pi N(0, 50 pi), height U(0.1, w/5), tangential speed
+ * sqrt(3) w / 4 plus a six-fold arm term inside w/2, masses (w^3/n) * (0.7 inside
+ * w/10, else 0.5). style 1 (lines 92-205): bulge, disc radius, thickness, arm
+ * count 2..6, arm tightness and a stable/energetic velocity range drawn once;
+ * per body radius disc * sqrt(u), Gaussian height, circular speed capped at 1.5x,
+ * a logarithmic-spiral radial perturbation and Gaussian dispersion. -1 draws the
+ * style from the seed (the reference's 50/50 choice). */
+int nb_ic_galaxy(uint64_t seed, int64_t n, double world_size, int style, double* pos3,
+                 double* vel3, float* mass);
+
 /* Reference particle-file format (ParticleSystemFile.cpp:10-90):
  *   "Particle System with N particles:" then per body "Particle ID: i",
  *   "Position: (x, y, z)", "Velocity: (...)", "Acceleration: (...)", "Mass: m".

@@ -20,7 +20,18 @@ struct Rng {
   }
   // uniform in (0, 1): 53 random bits, offset by half an ulp so 0 never appears
   double uniform() { return ((double)(next() >> 11) + 0.5) * (1.0 / 9007199254740992.0); }
+  double uniform(double lo, double hi) { return lo + (hi - lo) * uniform(); }
+  // standard normal deviate (Box & Muller 1958), one per call
+  double normal() { return std::sqrt(-2.0 * std::log(uniform())) * std::cos(2.0 * kPi * uniform()); }
 };
+
+// Velocity and mass of the cube and cube-surface initializers: each component
+// uniform in [-L, L] with L = |(w, w, w)| / 20, and mass (w^3 / n) * 1.2 * U(0.8, 1.3).
+void cube_velocity_mass(Rng& r, double w, int64_t n, double* vel3, float* mass) {
+  const double len = std::sqrt(3.0) * w / 20.0;
+  for (int k = 0; k < 3; ++k) vel3[k] = r.uniform(-len, len);
+  *mass = (float)(w * w * w / (double)n * 1.2 * r.uniform(0.8, 1.3));
+}
 
 void isotropic(Rng& r, double len, double out[3]) {
   const double z = 2.0 * r.uniform() - 1.0;
@@ -159,6 +170,152 @@ int nb_ic_lagrange(double world_size, double* pos3, double* vel3, float* mass) {
     vel3[3 * b + 1] = cy / cl * speed * 0.5f;
     vel3[3 * b + 2] = 0.0;
     mass[b] = m;
+  }
+  return 0;
+}
+
+int nb_ic_cube(uint64_t seed, int64_t n, double world_size, double* pos3, double* vel3,
+               float* mass) {
+  if (n < 1 || !pos3 || !vel3 || !mass || !(world_size > 0)) return -1;
+  // ParticleSystemCubeInitializer.cpp:12-26 (the 3-D branch of line 23-24).
+  Rng r(seed);
+  const double w = world_size;
+  for (int64_t i = 0; i < n; ++i) {
+    for (int k = 0; k < 3; ++k) pos3[3 * i + k] = r.uniform(0.0, w);
+    cube_velocity_mass(r, w, n, &vel3[3 * i], &mass[i]);
+  }
+  return 0;
+}
+
+int nb_ic_cube_surface(uint64_t seed, int64_t n, double world_size, double* pos3, double* vel3,
+                       float* mass) {
+  if (n < 1 || !pos3 || !vel3 || !mass || !(world_size > 0)) return -1;
+  // ParticleSystemCubeSurface.cpp:20-45: a face chosen uniformly, the other two
+  // coordinates uniform over it, the cube [0, w]^3.
+  Rng r(seed);
+  const double w = world_size;
+  for (int64_t i = 0; i < n; ++i) {
+    const int face = (int)(6.0 * r.uniform());  // 0..5: -x, +x, -y, +y, -z, +z
+    const int axis = face >> 1;
+    for (int k = 0; k < 3; ++k)
+      pos3[3 * i + k] = k == axis ? ((face & 1) ? w : 0.0) : r.uniform(0.0, w);
+    cube_velocity_mass(r, w, n, &vel3[3 * i], &mass[i]);
+  }
+  return 0;
+}
+
+int nb_ic_sphere_surface(uint64_t seed, int64_t n, double world_size, int pattern, double* pos3,
+                         double* vel3, float* mass) {
+  if (n < 1 || !pos3 || !vel3 || !mass || !(world_size > 0) || pattern < -1 || pattern > 2)
+    return -1;
+  // ParticleSystemSphere.cpp:28-130.
+  Rng r(seed);
+  const double w = world_size;
+  const double diag = std::sqrt(3.0) * w;
+  const double radius = diag / 3.0, c = 0.5 * w;
+  const double base_mass = w * w * w / (double)n * 0.5;
+  const double vbase = diag / 15.0;
+  if (pattern < 0) pattern = (int)(3.0 * r.uniform());  // one pattern for the whole system
+  const double an = std::sqrt(0.2 * 0.2 + 1.0 + 0.1 * 0.1);
+  const double axis[3] = {0.2 / an, 1.0 / an, -0.1 / an};
+  for (int64_t i = 0; i < n; ++i) {
+    double d[3];
+    isotropic(r, 1.0, d);  // uniform direction: the surface point is c + radius * d
+    for (int k = 0; k < 3; ++k) pos3[3 * i + k] = c + radius * d[k];
+    mass[i] = (float)(base_mass * r.uniform(0.7, 1.3));
+    const double speed = vbase * r.uniform(0.5, 1.5);
+    double* v = &vel3[3 * i];
+    if (pattern == 0 || pattern == 1) {  // outward explosion / inward implosion
+      const double s = pattern == 0 ? speed : -speed;
+      for (int k = 0; k < 3; ++k) v[k] = s * d[k];
+    } else {  // swirl about the fixed axis: direction axis x d
+      const double t[3] = {axis[1] * d[2] - axis[2] * d[1], axis[2] * d[0] - axis[0] * d[2],
+                           axis[0] * d[1] - axis[1] * d[0]};
+      const double tl = std::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+      if (tl * radius > 1e-6) {
+        for (int k = 0; k < 3; ++k) v[k] = speed * t[k] / tl;
+      } else {  // on the axis: a small random velocity (lines 106-115)
+        double rd[3];
+        isotropic(r, 0.1 * speed, rd);
+        for (int k = 0; k < 3; ++k) v[k] = rd[k];
+      }
+    }
+  }
+  return 0;
+}
+
+int nb_ic_galaxy(uint64_t seed, int64_t n, double world_size, int style, double* pos3,
+                 double* vel3, float* mass) {
+  if (n < 1 || !pos3 || !vel3 || !mass || !(world_size > 0) || style < -1 || style > 1)
+    return -1;
+  Rng r(seed);
+  const double w = world_size;
+  const double c = 0.5 * w;
+  const double vol = w * w * w;
+  if (style < 0) style = r.uniform() < 0.5 ? 0 : 1;  // the 50/50 choice of line 21-22
+  if (style == 0) {
+    // ParticleSystemGalaxyInitializer.cpp:46-91: radius uniform in [0, w), angle
+    // 2*pi*N(0, 50*pi), height U(0.1, w/5) above the centre plane, tangential speed
+    // |(w, w, w)|/4 with a six-fold arm term inside w/2.
+    const double len = std::sqrt(3.0) * w / 4.0;
+    const double bulge = w / 10.0, arm = w / 2.0;
+    for (int64_t i = 0; i < n; ++i) {
+      const double rad = r.uniform(0.0, w);
+      const double th = r.normal() * (3.1415 * 50.0) * 2.0 * kPi;
+      const double h = r.uniform(0.1, w / 5.0);
+      const double cs = std::cos(th), sn = std::sin(th);
+      double* p = &pos3[3 * i];
+      double* v = &vel3[3 * i];
+      p[0] = c + rad * cs;
+      p[1] = c + rad * sn;
+      p[2] = c + h;
+      mass[i] = (float)(vol / (double)n * (rad < bulge ? 0.7 : 0.5));
+      v[0] = -len * sn;
+      v[1] = len * cs;
+      v[2] = 0.0;
+      if (rad < arm) {
+        const double ang = std::atan2(p[1] - c, p[0] - c);
+        const double a = len * (rad / arm) * std::cos(6.0 * (ang - th));
+        v[0] -= a * cs;
+        v[1] -= a * sn;
+      }
+    }
+    return 0;
+  }
+  // ParticleSystemGalaxyInitializer.cpp:92-205: structure drawn once per system,
+  // then per body a centre-concentrated radius, a Gaussian layer, a capped
+  // circular speed, a logarithmic-spiral radial perturbation and dispersion.
+  const bool stable = r.uniform() < 0.7;
+  const double bulge = w * r.uniform(0.05, 0.15);
+  const double disc = w * r.uniform(0.3, 0.5);
+  const double thick = disc * r.uniform(0.02, 0.10);
+  const double arms = (double)(2 + (int)(5.0 * r.uniform()));  // 2..6
+  const double tight = r.uniform(0.5, 2.0);
+  const double vs_lo = stable ? 0.5 : 0.8, vs_hi = stable ? 0.9 : 1.5;
+  const double as_lo = stable ? 0.05 : 0.1, as_hi = stable ? 0.2 : 0.5;
+  const double disp_f = stable ? 0.05 : 0.15;
+  const double vbase = 0.5 * disc * r.uniform(vs_lo, vs_hi);
+  const double sigma = vbase * disp_f;
+  const double base_mass = vol / (double)n;
+  for (int64_t i = 0; i < n; ++i) {
+    const double rad = disc * std::sqrt(r.uniform());
+    const double th = r.uniform(0.0, 2.0 * kPi);
+    const double h = r.normal() * thick / 3.0;
+    const double cs = std::cos(th), sn = std::sin(th);
+    double* p = &pos3[3 * i];
+    double* v = &vel3[3 * i];
+    p[0] = c + rad * cs;
+    p[1] = c + rad * sn;
+    p[2] = c + h;
+    mass[i] = (float)(base_mass * (rad < bulge ? 1.5 : 1.0) * r.uniform(0.8, 1.2));
+    const double vmag = vbase * std::fmin(rad / (bulge + 1e-6), 1.5);
+    const double strength = r.uniform(as_lo, as_hi);
+    const double phase = tight * std::log(std::fmax(rad / (bulge + 1e-6), 1.0));
+    const double pert = strength * vmag * std::cos(arms * (th - phase));
+    const double radial = rad > 1e-6 ? 1.0 : 0.0;  // (cos, sin) is the unit radial direction
+    v[0] = -vmag * sn + radial * pert * cs + sigma * r.normal();
+    v[1] = vmag * cs + radial * pert * sn + sigma * r.normal();
+    v[2] = 0.5 * sigma * r.normal();
   }
   return 0;
 }

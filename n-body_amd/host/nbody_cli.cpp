@@ -6,8 +6,11 @@
 //   -v 3|4   direct sum (ParticleSolverHIP; the reference's v3/v4 GPU solvers)
 //   -v 7     Barnes-Hut (ParticleSolverBHutHIP; the reference's v7)
 //   -n N     number of bodies          -t dt     time step
-//   -i 3|5|8|9  Lagrange triangle, uniform ball (R=100, m=1), Plummer sphere
-//            (a=50, M=1e4), two colliding discs  — all seeded (--seed)
+//   -i 1..6   the reference's initializers in its world cube of edge 5
+//            (main.cpp:29): 1 cube, 2 disc galaxy (default), 3 Lagrange triangle,
+//            4 sphere surface, 5 uniform ball (R=100, m=1), 6 cube surface
+//   -i 8|9   Plummer sphere (a=50, M=1e4), two colliding discs
+//            every random initializer is seeded (--seed)
 //   -s eps2  squared softening         -f path   particle file (reference format)
 //   --steps K  steps to run (default 100)        --theta T / --leaf L (v7)
 //   --out path   write the final state in the reference format (full precision)
@@ -33,11 +36,13 @@
 namespace {
 
 [[noreturn]] void usage(const char* argv0, const char* why) {
-  std::fprintf(stderr, "%s\nusage: %s [-v 4|7] [-n N] [-i 3|5|8|9] [-t dt] [-s eps2] [-f file] "
+  std::fprintf(stderr, "%s\nusage: %s [-v 4|7] [-n N] [-i 1..6|8|9] [-t dt] [-s eps2] [-f file] "
                "[--steps K] [--seed S] [--theta T] [--leaf L] [--out file] [--snapshot file] "
                "[--resume file] [--energy] [--reorder] [--frames K --frame-out prefix]\n", why, argv0);
   std::exit(EXIT_FAILURE);
 }
+
+constexpr double kWorld = 5.0;  // the reference's world cube edge (main.cpp:29)
 
 std::vector<Particle> generate(int init, int64_t n, uint64_t seed, float eps2) {
   std::vector<double> p(3 * n), v(3 * n);
@@ -49,9 +54,13 @@ std::vector<Particle> generate(int init, int64_t n, uint64_t seed, float eps2) {
       p.resize(9);
       v.resize(9);
       m.resize(3);
-      rc = nb_ic_lagrange(5.0, p.data(), v.data(), m.data());  // world 5^3 (main.cpp:29)
+      rc = nb_ic_lagrange(kWorld, p.data(), v.data(), m.data());
       break;
+    case 1: rc = nb_ic_cube(seed, n, kWorld, p.data(), v.data(), m.data()); break;
+    case 2: rc = nb_ic_galaxy(seed, n, kWorld, -1, p.data(), v.data(), m.data()); break;
+    case 4: rc = nb_ic_sphere_surface(seed, n, kWorld, -1, p.data(), v.data(), m.data()); break;
     case 5: rc = nb_ic_ball(seed, n, 100.0, 1.0, p.data(), v.data(), m.data()); break;
+    case 6: rc = nb_ic_cube_surface(seed, n, kWorld, p.data(), v.data(), m.data()); break;
     case 8: rc = nb_ic_plummer(seed, n, 50.0, 1e4, 10.0, 1.0, p.data(), v.data(), m.data()); break;
     case 9:
       rc = nb_ic_discs(seed, n, 30.0, 2.0, 1e4, 1e4, 300.0, 0.5 / 0.0035, eps2, 1.0, p.data(),
@@ -93,9 +102,9 @@ std::vector<Particle> read_file(const std::string& path) {
 }  // namespace
 
 int main(int argc, char** argv) {
-  int version = 4, init = 5;
-  int64_t n = 100;                       // ArgumentsParser.cpp:8
-  float dt = 0.001f, eps2 = 0.0935f;     // ArgumentsParser.cpp:9-10
+  int version = 4, init = 2;             // init GALAXY (ArgumentsParser.cpp:7)
+  int64_t n = 100;                       // ArgumentsParser.cpp:6
+  float dt = 0.001f, eps2 = 0.0935f;     // ArgumentsParser.cpp:8-9
   int steps = 100, leaf = 0;
   float theta = 0.5f;
   uint64_t seed = 1;
@@ -133,6 +142,8 @@ int main(int argc, char** argv) {
   if (version == 1 || version == 2 || version == 5 || version == 6)
     usage(argv[0], "CPU solvers (-v 1, 2, 5, 6) are not part of this engine");
   if (version != 3 && version != 4 && version != 7) usage(argv[0], "Invalid version");
+  if (!(init >= 1 && init <= 6) && init != 8 && init != 9)
+    usage(argv[0], "Invalid initialization type");
 
   std::vector<Particle> particles = file.empty() ? generate(init, n, seed, eps2) : read_file(file);
   ParticleSystem system(particles, true);

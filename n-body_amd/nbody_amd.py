@@ -232,6 +232,10 @@ _SIGS = {
         [C.c_uint64, C.c_int64] + [C.c_double] * 8 + [_P, _P, _P],
     ),
     "nb_ic_lagrange": (C.c_int, [C.c_double, _P, _P, _P]),
+    "nb_ic_cube": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, _P, _P, _P]),
+    "nb_ic_cube_surface": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, _P, _P, _P]),
+    "nb_ic_sphere_surface": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, C.c_int, _P, _P, _P]),
+    "nb_ic_galaxy": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, C.c_int, _P, _P, _P]),
     "nb_ic_read_file": (C.c_int64, [C.c_char_p, C.c_int64, _P, _P, _P, _P]),
     "nb_ic_write_file": (C.c_int, [C.c_char_p, C.c_int64, _P, _P, _P, _P]),
     "nb_ic_log_uniform_sizes": (C.c_int, [C.c_uint64, C.c_int32, C.c_int32, C.c_int32, _P]),
@@ -322,6 +326,36 @@ def ic_lagrange(world_size: float = 5.0):
     m = np.empty(3, np.float32)
     _check(load().nb_ic_lagrange(world_size, _ptr(pos), _ptr(vel), _ptr(m)))
     return pos, vel, m
+
+
+def _ic_world(fn, seed, n, world_size, *mode):
+    pos = np.empty((n, 3), np.float64)
+    vel = np.empty((n, 3), np.float64)
+    m = np.empty(n, np.float32)
+    _check(fn(seed, n, world_size, *mode, _ptr(pos), _ptr(vel), _ptr(m)))
+    return pos, vel, m
+
+
+def ic_cube(seed: int, n: int, world_size: float = 5.0):
+    """Reference -i 1 (ParticleSystemCubeInitializer.cpp:8-29), seeded."""
+    return _ic_world(load().nb_ic_cube, seed, n, world_size)
+
+
+def ic_cube_surface(seed: int, n: int, world_size: float = 5.0):
+    """Reference -i 6 (ParticleSystemCubeSurface.cpp:8-49), seeded."""
+    return _ic_world(load().nb_ic_cube_surface, seed, n, world_size)
+
+
+def ic_sphere_surface(seed: int, n: int, world_size: float = 5.0, pattern: int = -1):
+    """Reference -i 4 (ParticleSystemSphere.cpp:20-133), seeded; pattern 0 outward,
+    1 inward, 2 swirl, -1 drawn from the seed."""
+    return _ic_world(load().nb_ic_sphere_surface, seed, n, world_size, pattern)
+
+
+def ic_galaxy(seed: int, n: int, world_size: float = 5.0, style: int = -1):
+    """Reference -i 2 (ParticleSystemGalaxyInitializer.cpp:12-210), seeded; style 0 or
+    1 selects the reference's two branches, -1 draws it from the seed."""
+    return _ic_world(load().nb_ic_galaxy, seed, n, world_size, style)
 
 
 def read_particle_file(path):

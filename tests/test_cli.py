@@ -19,7 +19,9 @@ def test_cpu_versions_are_rejected(v):
 
 @pytest.mark.parametrize("args,msg", [(("-v", 9), "Invalid version"), (("-n", 0), "Invalid number"),
                                       (("--bogus",), "unknown argument"), (("-t",), "missing value"),
-                                      (("--frames", 2), "go together")])
+                                      (("--frames", 2), "go together"),
+                                      (("-i", 7), "Invalid initialization"),
+                                      (("-i", 0), "Invalid initialization")])
 def test_bad_arguments(args, msg):
     r = run(*args)
     assert r.returncode != 0 and msg in r.stderr
