@@ -13,9 +13,12 @@ Also reported on the same line:
   cpu_baseline  the oracle's v2 restatement (OpenMP, the reference's flags) on a
                 bounded sample of the same workload on this host;
   bh_4m         Barnes-Hut step time on two colliding exponential discs,
-                N = 2^22, theta = 0.5 (skip with --no-bh);
+                N = 2^22, theta = 0.5 (skip with --no-bh); its roofline object
+                prices the traversal kernel at 20 flop per interaction counted on
+                the first timed step, over the mean traversal launch time;
   batch_c4      512 independent Plummer clusters (N_c log-uniform in
-                [256, 32768]) in one persistent launch per step (--no-batch);
+                [256, 32768]) in one persistent launch per step (--no-batch),
+                with the same roofline object for the batched force kernel;
   serve         with --serve S: the Poisson job stream through the continuous-
                 batching server and one job at a time, S seconds each.
 """
@@ -169,6 +172,14 @@ def force_errors(pos, mass, force, targets):
             "max": float(e.max())}
 
 
+def valu_roofline(kernel: str, interactions: float, ms: float):
+    """FP32 vector-ALU roofline of a force kernel: 20 flop per counted interaction
+    over its mean launch time (HIP events on its own stream)."""
+    achieved = 20.0 * interactions / (ms / 1e3) / 1e12
+    return {"kernel": kernel, "bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS}
+
+
 def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0, accuracy: bool = True):
     n = 1 << 22
     t0 = time.perf_counter()
@@ -208,6 +219,7 @@ def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0, accuracy: boo
         "traverse_ms": t["force_ms"] / max(1, t["force_calls"]),
         "interactions_first_step": inter,
         "interactions_per_s": inter / (t["force_ms"] / max(1, t["force_calls"]) / 1e3),
+        "roofline": valu_roofline("k_traverse_list", inter, t["force_ms"] / max(1, t["force_calls"])),
         "force_error_vs_fp64": acc,
         "cpu_baseline": cpu,
     }
@@ -256,6 +268,7 @@ def bench_batch(nb, steps: int, warmup: int, energy: bool, tune: bool = True):
             "ms_per_step": 1e3 * wall / steps,
             "force_ms": t["force_ms"] / max(1, t["force_calls"]),
             "interactions_per_s": inter * steps / wall,
+            "roofline": valu_roofline("k_batch_force", inter, t["force_ms"] / max(1, t["force_calls"])),
             "shape": {"bodies_per_lane": shape["bodies_per_lane"], "j_split": shape["j_split"],
                       "tuned": True} if shape else "built-in 2x8",
         }
