@@ -80,9 +80,12 @@ typedef struct nb_config {
 
 #define NB_F_NO_GRAPH 0x1 /* launch steps eagerly instead of replaying a captured hipGraph */
 /* Barnes-Hut: open cells per body, exactly the reference's test (Octree.cpp:320),
- * instead of the default per-wave test on the bounding box of 64 Morton-consecutive
- * bodies (never less accurate per body, no divergence). */
+ * instead of the default per-group test on the bounding box of at most
+ * NB_BH_WAVE_BODIES Morton-consecutive bodies of one parent cell (never less
+ * accurate per body, no divergence). */
 #define NB_F_BH_PER_BODY 0x4
+/* Largest group of bodies that share one walk in the default Barnes-Hut traversal. */
+#define NB_BH_WAVE_BODIES 32
 /* Barnes-Hut: quantize each axis over its own extent, exactly the inputs of the
  * reference's morton.glsl:68-75 (cells become boxes with the root's aspect
  * ratio), instead of the default cubic root cell (extent = the largest axis). */

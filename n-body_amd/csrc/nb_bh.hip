@@ -56,8 +56,14 @@ constexpr int kSpare = 4;  // zero-mass bodies after the sorted body array
 // handled at its own level; a level-10 leaf that holds more is split into
 // chunks), so every range lies inside one parent cell and its bounding box
 // stays tight. (With leaf_size <= kWaveBodies only level-10 leaves can hold
-// more than kWaveBodies bodies.)
-constexpr int kWaveBodies = 64;
+// more than kWaveBodies bodies.) A wave holds 64 / P copies of its range (P = the
+// range's body count rounded up to a power of two), so at 32 bodies every range
+// is evaluated by at least two slices: against 64-body ranges this measured 8%
+// fewer interactions and a 4% faster traversal on the 4M two-disc system (16
+// bodies: 14% slower, the walks dominate).
+constexpr int kWaveBodies = NB_BH_WAVE_BODIES;
+static_assert(kWaveBodies >= 4 && kWaveBodies <= 64 && (kWaveBodies & (kWaveBodies - 1)) == 0,
+              "a wave range fits one wave");
 
 // ---- 1. bounding box --------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bbox_partial(const float4* __restrict__ pm, int n,

@@ -35,6 +35,7 @@ INTEG_EULER = 1
 F_NO_GRAPH = 0x1
 F_BH_PER_BODY = 0x4
 F_BH_AXIS_BOX = 0x8
+BH_WAVE_BODIES = 32  # NB_BH_WAVE_BODIES
 
 
 class NBodyError(RuntimeError):

@@ -63,8 +63,9 @@ def test_bit_exact_topology_and_forces(nb, ora, leaf):
     np.testing.assert_array_equal(nodes["level"], tree["level"])
     np.testing.assert_allclose(nodes["mass"], com[:, 3], rtol=1e-6)
     np.testing.assert_allclose(nodes["com"], com[:, :3], rtol=1e-6, atol=1e-5)
-    # default wave traversal vs its CPU twin (groups of 64 sorted bodies)
-    fc, cinter = ora.bh_forces_group(p4[cperm], cperm, mn, mx, tree, com, EPS2, group=64)
+    # default wave traversal vs its CPU twin (groups of NB_BH_WAVE_BODIES sorted bodies)
+    fc, cinter = ora.bh_forces_group(p4[cperm], cperm, mn, mx, tree, com, EPS2,
+                                     group=nb.BH_WAVE_BODIES)
     e = rel_err(f, fc)
     print(f"leaf {leaf}: GPU vs CPU build p50={np.median(e):.2e} p99={np.percentile(e, 99):.2e}; "
           f"interactions {inter} vs {cinter}")
@@ -137,7 +138,7 @@ def test_steps_kdk_vs_cpu_pipeline(nb, ora):
     op, ov, oa = pos.copy(), vel.copy(), np.zeros_like(pos)
     for _ in range(5):
         f, *_ = ora.bh_pipeline(pm4(op.astype(np.float32), m), EPS2, theta=0.5, leaf_size=8,
-                                group=64, cubic=True)
+                                group=nb.BH_WAVE_BODIES, cubic=True)
         ora.kdk_update_f64(op, ov, oa, f[:, :3].astype(np.float64), DT)
     assert np.abs(p - op).max() < 1e-6 * np.abs(op).max()
     assert rel_err(a, oa).max() < 1e-3
@@ -197,7 +198,8 @@ def test_axis_box_matches_reference_quantisation(nb, ora):
     np.testing.assert_array_equal(codes, ccodes)
     np.testing.assert_array_equal(perm, cperm)
     np.testing.assert_array_equal(nodes["next"], tree["next"])
-    fc, cinter = ora.bh_forces_group(p4[cperm], cperm, mn, mx, tree, com, EPS2, group=64)
+    fc, cinter = ora.bh_forces_group(p4[cperm], cperm, mn, mx, tree, com, EPS2,
+                                     group=nb.BH_WAVE_BODIES)
     assert np.percentile(rel_err(f, fc), 99) < 1e-5
     # the cubic default opens fewer cells on flat systems
     _, _, _, _, _, inter_cubic = gpu_run(nb, pos, vel, m, 8)
