@@ -204,3 +204,42 @@ def test_axis_box_matches_reference_quantisation(nb, ora):
     # the cubic default opens fewer cells on flat systems
     _, _, _, _, _, inter_cubic = gpu_run(nb, pos, vel, m, 8)
     assert inter_cubic < inter
+
+
+def test_largest_tested_size_2_24(nb, ora):
+    """N = 2^24 Plummer sphere (four times the C3 size): Morton codes, permutation
+    and octree topology bit-exact against the CPU build, the root mass equal to
+    the total mass, and theta=0.5 forces within the C3 accuracy on seeded targets.
+    Exercises the multi-block scans and sort at 4096 sort tiles."""
+    n = 1 << 24
+    pos, vel, m = nb.ic_plummer(2424, n, 50.0, 1e4)
+    print(f"N=2^24: uploading {n} bodies", flush=True)
+    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH, leaf_size=8) as sim:
+        sim.upload(pos, vel, m)
+        f = sim.forces_f64()
+        codes, perm = sim.debug_morton()
+        nodes, bbox = sim.debug_tree()
+    print(f"N=2^24: {len(nodes)} nodes; checking against the CPU build", flush=True)
+    p4 = pm4(pos.astype(np.float32), m)
+    mn, mx = ora.bbox(p4)
+    np.testing.assert_array_equal(bbox[:3], mn)
+    np.testing.assert_array_equal(bbox[3:], mx)
+    ccodes = ora.morton(p4, mn, mx, cubic=True)
+    np.testing.assert_array_equal(codes, ccodes)
+    cperm, srt = ora.sort_codes(ccodes)
+    np.testing.assert_array_equal(perm, cperm)
+    del ccodes, codes
+    tree = ora.build_tree(srt, 8)
+    assert len(nodes) == len(tree["start"])
+    np.testing.assert_array_equal(nodes["first_child"], tree["first_child"])
+    np.testing.assert_array_equal(nodes["next"], tree["next"])
+    np.testing.assert_array_equal(nodes["body_start"], tree["start"])
+    np.testing.assert_array_equal(nodes["body_count"], tree["count"])
+    np.testing.assert_array_equal(nodes["level"], tree["level"])
+    assert abs(nodes["mass"][0] - m.astype(np.float64).sum()) < 1e-5 * m.sum()
+    tgt = np.random.default_rng(6).choice(n, 64, replace=False)
+    ref = ora.direct_f64(pos.astype(np.float32).astype(np.float64), m.astype(np.float64), EPS2,
+                         targets=tgt)
+    e = rel_err(f[tgt], ref)
+    print(f"N=2^24 Plummer vs FP64 (64 targets): p50={np.median(e):.2e} p99={np.percentile(e, 99):.2e}")
+    assert np.median(e) < 1e-2
