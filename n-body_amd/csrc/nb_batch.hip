@@ -93,7 +93,7 @@ __global__ __launch_bounds__(64 * K) void k_batch_force(
     const int ibase = off + it.y * (64 * B) + lane;
     nbcore::Targets<B> t;
     nbcore::load_targets<B>(pm, ibase, t);
-    const int jlen = npad / K;  // npad is a multiple of 256: a multiple of 4 for K <= 64
+    const int jlen = npad / K;  // npad is a multiple of 256: a multiple of 16 for K <= 16
     const int j0 = off + wave * jlen;
     if (eps2 >= 1e-20f)
       nbcore::accumulate<B, false>(pm, j0, j0 + jlen, eps2, t);

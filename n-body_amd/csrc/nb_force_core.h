@@ -29,7 +29,7 @@ struct Targets {
   float cx[B], cy[B], cz[B];  // their error terms
 };
 
-// Accumulate the sources [j0, j1) (wave-uniform, j1 - j0 a multiple of 4).
+// Accumulate the sources [j0, j1) (wave-uniform, j1 - j0 a multiple of 16).
 template <int B, bool kGuard>
 __device__ __forceinline__ void accumulate(const float4* __restrict__ pm, int j0, int j1,
                                            float eps2, Targets<B>& t) {
@@ -38,10 +38,10 @@ __device__ __forceinline__ void accumulate(const float4* __restrict__ pm, int j0
     float px[B], py[B], pz[B];
 #pragma unroll
     for (int b = 0; b < B; ++b) px[b] = py[b] = pz[b] = 0.f;
-    for (int j = jc; j < je; j += 4) {
+    auto step = [&](const float4* qq) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const float4 q = pm[j + u];  // wave-uniform address: scalar load
+        const float4 q = qq[u];
 #pragma unroll
         for (int b = 0; b < B; ++b) {
           const float dx = q.x - t.x[b];
@@ -56,6 +56,26 @@ __device__ __forceinline__ void accumulate(const float4* __restrict__ pm, int j0
           pz[b] = fmaf(dz, s, pz[b]);
         }
       }
+    };
+    // Two register sets of four sources: the scalar loads of one set are
+    // issued before the other set's arithmetic, so the load latency overlaps
+    // the math instead of stalling every four sources (measured 10% faster at
+    // N = 2^18 than loading each set right before its use). The memory clobber
+    // keeps the compiler from sinking the loads next to their first use.
+    // je - jc is a multiple of 8 (ranges are multiples of 16 sources).
+    float4 qa[4], qb[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) qa[u] = pm[jc + u];
+    for (int j = jc; j < je; j += 8) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) qb[u] = pm[j + 4 + u];  // wave-uniform: scalar loads
+      asm volatile("" ::: "memory");
+      step(qa);
+      const int jn = j + 8 < je ? j + 8 : j;  // the last step reloads its own (in range)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) qa[u] = pm[jn + u];
+      asm volatile("" ::: "memory");
+      step(qb);
     }
 #pragma unroll
     for (int b = 0; b < B; ++b) {
