@@ -19,8 +19,10 @@ Also reported on the same line:
   batch_c4      512 independent Plummer clusters (N_c log-uniform in
                 [256, 32768]) in one persistent launch per step (--no-batch),
                 with the same roofline object for the batched force kernel;
-  serve         with --serve S: the Poisson job stream through the continuous-
-                batching server and one job at a time, S seconds each.
+  serve         the Poisson job stream (50 jobs/s) through the continuous-
+                batching server (look-ahead 16 and 4096) and one job at a time,
+                S seconds each (--serve S, default 20; 0 skips it): jobs and
+                body-steps finished per second and p50/p99 job latency.
 """
 from __future__ import annotations
 
@@ -456,8 +458,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--batch-energy", action="store_true", help="per-cluster FP64 energy drift")
-    ap.add_argument("--serve", type=float, default=0.0,
-                    help="also run the serving stream for this many seconds per mode")
+    ap.add_argument("--serve", type=float, default=20.0,
+                    help="seconds of the serving stream per mode (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=50.0)
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner (built-in shapes)")
     ap.add_argument("--no-accuracy", action="store_true",
@@ -488,6 +490,7 @@ def main():
         return
     if args.only == "direct":
         args.no_bh = args.no_batch = args.no_cpu = True
+        args.serve = 0.0
     n = args.n
     tuning = None
     if not args.no_tune:
