@@ -530,8 +530,14 @@ int nb_batch_step(nb_batch* b, int32_t nsteps) {
       if (g) (void)hipGraphDestroy(g);
       return nb_fail(NB_E_HIP, "nb_batch_step: graph capture failed");
     }
+    const hipError_t ei = hipGraphInstantiate(&b->graph_exec, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+      (void)hipGraphDestroy(g);
+      b->graph_exec = nullptr;
+      return nb_fail(NB_E_HIP, std::string("nb_batch_step: graph instantiation failed: ") +
+                                   hipGetErrorString(ei));
+    }
     b->graph = g;
-    NB_HIP(hipGraphInstantiate(&b->graph_exec, b->graph, nullptr, nullptr, 0));
   }
   for (int s = 0; s < nsteps; ++s) NB_HIP(hipGraphLaunch(b->graph_exec, b->stream));
   return NB_OK;

@@ -331,8 +331,14 @@ int nb_step(nb_ctx* c, int32_t nsteps) {
         return nb_fail(NB_E_HIP, std::string("nb_step: graph capture failed: ") +
                                      hipGetErrorString(e != hipSuccess ? e : e2));
       }
+      const hipError_t ei = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+      if (ei != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        c->graph_exec = nullptr;
+        return nb_fail(NB_E_HIP, std::string("nb_step: graph instantiation failed: ") +
+                                     hipGetErrorString(ei));
+      }
       c->graph = g;
-      NB_HIP(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
     }
     for (int s = 0; s < nsteps; ++s) NB_HIP(hipGraphLaunch(c->graph_exec, c->stream));
   }
