@@ -43,7 +43,7 @@ extern "C" {
 #define NB_OK 0
 #define NB_E_ARG (-1)           /* bad argument / size / state */
 #define NB_E_HIP (-2)           /* HIP runtime error */
-#define NB_E_OOM (-3)           /* device allocation failed */
+#define NB_E_OOM (-3)           /* device (or host) allocation failed */
 #define NB_E_BUDGET (-4)        /* request exceeds the HBM admission budget */
 #define NB_E_TREE_OVERFLOW (-5) /* octree node pool exhausted */
 #define NB_E_NONFINITE (-6)     /* a NaN/Inf appeared in the particle state */

@@ -10,7 +10,8 @@
  * (0, 1), so CPU checks and GPU runs start from identical bytes.
  *
  * Outputs: pos3/vel3 are double[3n] (x, y, z interleaved), mass is float[n].
- * All return 0 on success or NB_E_ARG (-1).
+ * All return 0 on success or NB_E_ARG (-1); the particle-file functions return
+ * NB_E_OOM (-3) when a host allocation fails.
  */
 #ifndef NBODY_IC_H
 #define NBODY_IC_H

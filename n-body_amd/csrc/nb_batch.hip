@@ -310,6 +310,7 @@ int check_slot(nb_batch* b, int32_t id) {
 extern "C" {
 
 int nb_batch_create(nb_batch** out, const nb_batch_config* cfg) {
+  NB_TRY
   NB_API_SHARED;
   if (!out || !cfg) return nb_fail(NB_E_ARG, "nb_batch_create: null argument");
   *out = nullptr;
@@ -377,6 +378,7 @@ int nb_batch_create(nb_batch** out, const nb_batch_config* cfg) {
   b->free_list[0] = b->cap;
   *out = b;
   return NB_OK;
+  NB_CATCH("nb_batch_create")
 }
 
 void nb_batch_destroy(nb_batch* b) {
@@ -388,6 +390,7 @@ void nb_batch_destroy(nb_batch* b) {
 
 int nb_batch_add(nb_batch* b, const double* pos3, const double* vel3, const double* acc3,
                  const float* mass, int64_t n, float eps2, float dt, float G, int32_t* id) {
+  NB_TRY
   NB_API_SHARED;
   if (!b || !pos3 || !vel3 || !mass || !id) return nb_fail(NB_E_ARG, "nb_batch_add: null argument");
   if (n < 1) return nb_fail(NB_E_ARG, "nb_batch_add: n < 1");
@@ -457,9 +460,11 @@ int nb_batch_add(nb_batch* b, const double* pos3, const double* vel3, const doub
   b->dirty = true;
   *id = slot;
   return NB_OK;
+  NB_CATCH("nb_batch_add")
 }
 
 int nb_batch_remove(nb_batch* b, int32_t id) {
+  NB_TRY
   NB_API_SHARED;
   int rc = check_slot(b, id);
   if (rc) return rc;
@@ -488,9 +493,11 @@ int nb_batch_remove(nb_batch* b, int32_t id) {
   b->free_list[off] = len;
   b->dirty = true;
   return NB_OK;
+  NB_CATCH("nb_batch_remove")
 }
 
 int nb_batch_step(nb_batch* b, int32_t nsteps) {
+  NB_TRY
   if (!b) return nb_fail(NB_E_ARG, "nb_batch_step: null batch");
   if (nsteps < 0) return nb_fail(NB_E_ARG, "nb_batch_step: nsteps < 0");
   {
@@ -541,13 +548,16 @@ int nb_batch_step(nb_batch* b, int32_t nsteps) {
   }
   for (int s = 0; s < nsteps; ++s) NB_HIP(hipGraphLaunch(b->graph_exec, b->stream));
   return NB_OK;
+  NB_CATCH("nb_batch_step")
 }
 
 int nb_batch_sync(nb_batch* b) {
+  NB_TRY
   NB_API_SHARED;
   if (!b) return nb_fail(NB_E_ARG, "nb_batch_sync: null batch");
   NB_HIP(hipStreamSynchronize(b->stream));
   return NB_OK;
+  NB_CATCH("nb_batch_sync")
 }
 
 static int copy_slices(nb_batch* b, int32_t id, const double* src, double* dst) {
@@ -561,6 +571,7 @@ static int copy_slices(nb_batch* b, int32_t id, const double* src, double* dst) 
 }
 
 int nb_batch_forces(nb_batch* b, int32_t id, double* force3) {
+  NB_TRY
   NB_API_SHARED;
   int rc = check_slot(b, id);
   if (rc) return rc;
@@ -570,9 +581,11 @@ int nb_batch_forces(nb_batch* b, int32_t id, double* force3) {
   NB_HIP(enqueue_force(b));
   NB_HIP(hipStreamSynchronize(b->stream));
   return copy_slices(b, id, b->force, force3);
+  NB_CATCH("nb_batch_forces")
 }
 
 int nb_batch_download(nb_batch* b, int32_t id, double* pos3, double* vel3, double* acc3) {
+  NB_TRY
   NB_API_SHARED;
   int rc = check_slot(b, id);
   if (rc) return rc;
@@ -581,9 +594,11 @@ int nb_batch_download(nb_batch* b, int32_t id, double* pos3, double* vel3, doubl
   if (vel3 && (rc = copy_slices(b, id, b->vel, vel3))) return rc;
   if (acc3 && (rc = copy_slices(b, id, b->acc, acc3))) return rc;
   return NB_OK;
+  NB_CATCH("nb_batch_download")
 }
 
 int nb_batch_energy(nb_batch* b, int32_t id, double* ekin, double* epot) {
+  NB_TRY
   NB_API_SHARED;
   int rc = check_slot(b, id);
   if (rc) return rc;
@@ -609,9 +624,11 @@ int nb_batch_energy(nb_batch* b, int32_t id, double* ekin, double* epot) {
   if (ekin) *ekin = ek;
   if (epot) *epot = -0.5 * G * ep;
   return NB_OK;
+  NB_CATCH("nb_batch_energy")
 }
 
 int nb_batch_info(nb_batch* b, int32_t* clusters, int64_t* bodies, int64_t* interactions) {
+  NB_TRY
   if (!b) return nb_fail(NB_E_ARG, "nb_batch_info: null batch");
   int rc = rebuild_tables(b);
   if (rc) return rc;
@@ -619,25 +636,31 @@ int nb_batch_info(nb_batch* b, int32_t* clusters, int64_t* bodies, int64_t* inte
   if (bodies) *bodies = b->used_bodies;
   if (interactions) *interactions = b->interactions;
   return NB_OK;
+  NB_CATCH("nb_batch_info")
 }
 
 int nb_batch_set_timing(nb_batch* b, int32_t enable) {
+  NB_TRY
   NB_API_SHARED;
   if (!b) return nb_fail(NB_E_ARG, "nb_batch_set_timing: null batch");
   NB_HIP(hipStreamSynchronize(b->stream));
   b->timing = enable != 0;
   b->tim = nb_timing{};
   return NB_OK;
+  NB_CATCH("nb_batch_set_timing")
 }
 
 int nb_batch_get_timing(nb_batch* b, nb_timing* out) {
+  NB_TRY
   if (!b || !out) return nb_fail(NB_E_ARG, "nb_batch_get_timing: null argument");
   *out = b->tim;
   out->interactions = b->interactions;
   return NB_OK;
+  NB_CATCH("nb_batch_get_timing")
 }
 
 int nb_batch_tune(nb_batch* b, int32_t reps, nb_tuning* out) {
+  NB_TRY
   NB_API_SHARED;
   if (!b) return nb_fail(NB_E_ARG, "nb_batch_tune: null batch");
   if (reps < 1) reps = 3;
@@ -689,6 +712,7 @@ int nb_batch_tune(nb_batch* b, int32_t reps, nb_tuning* out) {
     out->direct_ms = best_ms;
   }
   return NB_OK;
+  NB_CATCH("nb_batch_tune")
 }
 
 }  // extern "C"

@@ -1307,6 +1307,7 @@ hipError_t iscan(const int* in, int* out, int* part, const int* level_off, int l
 }  // namespace
 
 int nb_sort_alloc(nb_ctx* c) {
+  NB_TRY
   nb_bh_state& b = c->bh;
   if (b.keys[0]) return NB_OK;
   const int64_t n = c->n;
@@ -1326,9 +1327,11 @@ int nb_sort_alloc(nb_ctx* c) {
   if (hipMemset(b.geom, 0, sizeof(float) * 32) != hipSuccess)
     return nb_fail(NB_E_HIP, "hipMemset failed (Morton sort)");
   return NB_OK;
+  NB_CATCH("nb_sort_alloc")
 }
 
 int nb_bh_alloc(nb_ctx* c) {
+  NB_TRY
   nb_bh_state& b = c->bh;
   const int64_t n = c->n;
   int64_t bound = 0, w = 1;
@@ -1362,6 +1365,7 @@ int nb_bh_alloc(nb_ctx* c) {
   if (hipMemset(b.pms + n, 0, sizeof(float4) * kSpare) != hipSuccess)
     return nb_fail(NB_E_HIP, "nb_create: hipMemset failed (octree)");
   return NB_OK;
+  NB_CATCH("nb_bh_alloc")
 }
 
 void nb_bh_free(nb_ctx* c) {
@@ -1500,6 +1504,7 @@ hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions) {
 
 // ---- debug taps ---------------------------------------------------------------------------------
 extern "C" int nb_debug_morton(nb_ctx* c, uint32_t* codes, uint32_t* perm) {
+  NB_TRY
   if (!c) return nb_fail(NB_E_ARG, "nb_debug_morton: null context");
   if (c->cfg.solver != NB_SOLVER_BH) return nb_fail(NB_E_ARG, "nb_debug_morton: not a BH context");
   if (!c->tree_valid) return nb_fail(NB_E_ARG, "nb_debug_morton: no BH force pass has run");
@@ -1507,10 +1512,12 @@ extern "C" int nb_debug_morton(nb_ctx* c, uint32_t* codes, uint32_t* perm) {
   if (codes) NB_HIP(hipMemcpy(codes, c->bh.codes, sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost));
   if (perm) NB_HIP(hipMemcpy(perm, c->bh.vals[0], sizeof(uint32_t) * c->n, hipMemcpyDeviceToHost));
   return NB_OK;
+  NB_CATCH("nb_debug_morton")
 }
 
 extern "C" int nb_debug_tree(nb_ctx* c, nb_node_view* nodes, int64_t cap, int64_t* node_count,
                              float* bbox) {
+  NB_TRY
   if (!c) return nb_fail(NB_E_ARG, "nb_debug_tree: null context");
   if (c->cfg.solver != NB_SOLVER_BH) return nb_fail(NB_E_ARG, "nb_debug_tree: not a BH context");
   if (!c->tree_valid) return nb_fail(NB_E_ARG, "nb_debug_tree: no BH force pass has run");
@@ -1541,4 +1548,5 @@ extern "C" int nb_debug_tree(nb_ctx* c, nb_node_view* nodes, int64_t cap, int64_
     }
   }
   return NB_OK;
+  NB_CATCH("nb_debug_tree")
 }

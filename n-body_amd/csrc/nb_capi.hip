@@ -18,7 +18,10 @@ static thread_local std::string g_last_error;
 
 void nb_set_error(const std::string& msg) { g_last_error = msg; }
 int nb_fail(int code, const std::string& msg) {
-  g_last_error = msg;
+  try {
+    g_last_error = msg;
+  } catch (...) {  // keep the code; the message is lost only when the copy cannot be made
+  }
   return code;
 }
 
@@ -162,6 +165,7 @@ static int accumulate_timing(nb_ctx* c) {
 extern "C" {
 
 int nb_create(nb_ctx** out, const nb_config* cfg) {
+  NB_TRY
   NB_API_SHARED;
   if (!out || !cfg) return nb_fail(NB_E_ARG, "nb_create: null argument");
   *out = nullptr;
@@ -240,6 +244,7 @@ int nb_create(nb_ctx** out, const nb_config* cfg) {
   }
   *out = c;
   return NB_OK;
+  NB_CATCH("nb_create")
 }
 
 void nb_destroy(nb_ctx* ctx) {
@@ -270,6 +275,7 @@ static int upload_common(nb_ctx* c, const std::vector<double>& p, const std::vec
 
 int nb_upload(nb_ctx* c, const float* pos4, const float* vel4, const float* acc4,
               const float* mass, int64_t n) {
+  NB_TRY
   NB_API_SHARED;
   if (!c || !pos4 || !vel4 || !mass) return nb_fail(NB_E_ARG, "nb_upload: null argument");
   if (n != c->n) return nb_fail(NB_E_ARG, "nb_upload: n differs from the context's n");
@@ -281,10 +287,12 @@ int nb_upload(nb_ctx* c, const float* pos4, const float* vel4, const float* acc4
       if (acc4) a[k * n + i] = acc4[4 * i + k];
     }
   return upload_common(c, p, v, a, mass);
+  NB_CATCH("nb_upload")
 }
 
 int nb_upload_f64(nb_ctx* c, const double* pos3, const double* vel3, const double* acc3,
                   const float* mass, int64_t n) {
+  NB_TRY
   NB_API_SHARED;
   if (!c || !pos3 || !vel3 || !mass) return nb_fail(NB_E_ARG, "nb_upload_f64: null argument");
   if (n != c->n) return nb_fail(NB_E_ARG, "nb_upload_f64: n differs from the context's n");
@@ -296,9 +304,11 @@ int nb_upload_f64(nb_ctx* c, const double* pos3, const double* vel3, const doubl
       if (acc3) a[k * n + i] = acc3[3 * i + k];
     }
   return upload_common(c, p, v, a, mass);
+  NB_CATCH("nb_upload_f64")
 }
 
 int nb_step(nb_ctx* c, int32_t nsteps) {
+  NB_TRY
   if (!c) return nb_fail(NB_E_ARG, "nb_step: null context");
   if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_step: no particle system uploaded");
   if (nsteps < 0) return nb_fail(NB_E_ARG, "nb_step: nsteps < 0");
@@ -344,13 +354,16 @@ int nb_step(nb_ctx* c, int32_t nsteps) {
   }
   c->tree_valid = c->cfg.solver == NB_SOLVER_BH;
   return NB_OK;
+  NB_CATCH("nb_step")
 }
 
 int nb_sync(nb_ctx* c) {
+  NB_TRY
   NB_API_SHARED;
   if (!c) return nb_fail(NB_E_ARG, "nb_sync: null context");
   NB_HIP(hipStreamSynchronize(c->stream));
   return check_flags(c);
+  NB_CATCH("nb_sync")
 }
 
 static int force_pass(nb_ctx* c) {
@@ -367,6 +380,7 @@ static int force_pass(nb_ctx* c) {
 }
 
 int nb_forces(nb_ctx* c, float* force4) {
+  NB_TRY
   NB_API_SHARED;
   if (!c || !force4) return nb_fail(NB_E_ARG, "nb_forces: null argument");
   int rc = force_pass(c);
@@ -382,9 +396,11 @@ int nb_forces(nb_ctx* c, float* force4) {
     force4[4 * i + 3] = 0.f;
   }
   return NB_OK;
+  NB_CATCH("nb_forces")
 }
 
 int nb_forces_f64(nb_ctx* c, double* force3) {
+  NB_TRY
   NB_API_SHARED;
   if (!c || !force3) return nb_fail(NB_E_ARG, "nb_forces_f64: null argument");
   int rc = force_pass(c);
@@ -396,6 +412,7 @@ int nb_forces_f64(nb_ctx* c, double* force3) {
   for (size_t i = 0; i < n; ++i)
     for (int k = 0; k < 3; ++k) force3[3 * i + k] = f[k * n + i];
   return NB_OK;
+  NB_CATCH("nb_forces_f64")
 }
 
 static int download_soa(nb_ctx* c, std::vector<double>& p, std::vector<double>& v,
@@ -415,6 +432,7 @@ static int download_soa(nb_ctx* c, std::vector<double>& p, std::vector<double>& 
 }
 
 int nb_download(nb_ctx* c, float* pos4, float* vel4, float* acc4) {
+  NB_TRY
   NB_API_SHARED;
   if (!c) return nb_fail(NB_E_ARG, "nb_download: null context");
   std::vector<double> p, v, a;
@@ -427,9 +445,11 @@ int nb_download(nb_ctx* c, float* pos4, float* vel4, float* acc4) {
       if (acc4) acc4[4 * i + k] = k < 3 ? (float)a[k * n + i] : 0.f;
     }
   return rc;
+  NB_CATCH("nb_download")
 }
 
 int nb_download_f64(nb_ctx* c, double* pos3, double* vel3, double* acc3) {
+  NB_TRY
   NB_API_SHARED;
   if (!c) return nb_fail(NB_E_ARG, "nb_download_f64: null context");
   std::vector<double> p, v, a;
@@ -442,9 +462,11 @@ int nb_download_f64(nb_ctx* c, double* pos3, double* vel3, double* acc3) {
       if (acc3) acc3[3 * i + k] = a[k * n + i];
     }
   return rc;
+  NB_CATCH("nb_download_f64")
 }
 
 int nb_energy(nb_ctx* c, double* ekin, double* epot, double p[3]) {
+  NB_TRY
   NB_API_SHARED;
   if (!c) return nb_fail(NB_E_ARG, "nb_energy: null context");
   if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_energy: no particle system uploaded");
@@ -466,6 +488,7 @@ int nb_energy(nb_ctx* c, double* ekin, double* epot, double p[3]) {
   if (epot) *epot = -0.5 * c->G * ep;
   if (p) { p[0] = px; p[1] = py; p[2] = pz; }
   return check_flags(c);
+  NB_CATCH("nb_energy")
 }
 
 // Snapshot layout (little-endian): "NBSNAP01", int64 n, float eps2, float dt,
@@ -475,6 +498,7 @@ int nb_energy(nb_ctx* c, double* ekin, double* epot, double p[3]) {
 static const char kSnapMagic[8] = {'N', 'B', 'S', 'N', 'A', 'P', '0', '1'};
 
 int nb_save_snapshot(nb_ctx* c, const char* path) {
+  NB_TRY
   NB_API_SHARED;
   if (!c || !path) return nb_fail(NB_E_ARG, "nb_save_snapshot: null argument");
   if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_save_snapshot: nothing uploaded");
@@ -497,9 +521,11 @@ int nb_save_snapshot(nb_ctx* c, const char* path) {
             std::fwrite(m.data(), 4, m.size(), f) == m.size();
   ok = (std::fclose(f) == 0) && ok;
   return ok ? NB_OK : nb_fail(NB_E_ARG, std::string("nb_save_snapshot: write failed: ") + path);
+  NB_CATCH("nb_save_snapshot")
 }
 
 int nb_load_snapshot(nb_ctx* c, const char* path) {
+  NB_TRY
   NB_API_SHARED;
   if (!c || !path) return nb_fail(NB_E_ARG, "nb_load_snapshot: null argument");
   FILE* f = std::fopen(path, "rb");
@@ -524,6 +550,7 @@ int nb_load_snapshot(nb_ctx* c, const char* path) {
   std::fclose(f);
   if (!ok) return nb_fail(NB_E_ARG, "nb_load_snapshot: truncated file");
   return upload_common(c, p, v, a, m.data());
+  NB_CATCH("nb_load_snapshot")
 }
 
 static int ensure_tmp(nb_ctx* c) {
@@ -535,6 +562,7 @@ static int ensure_tmp(nb_ctx* c) {
 }
 
 int nb_reorder(nb_ctx* c) {
+  NB_TRY
   NB_API_SHARED;
   if (!c) return nb_fail(NB_E_ARG, "nb_reorder: null context");
   if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_reorder: nothing uploaded");
@@ -556,9 +584,11 @@ int nb_reorder(nb_ctx* c) {
   c->order.swap(order);
   c->tree_valid = false;
   return check_flags(c);
+  NB_CATCH("nb_reorder")
 }
 
 int nb_render_view(nb_ctx* c, float* xyzs) {
+  NB_TRY
   NB_API_SHARED;
   if (!c || !xyzs) return nb_fail(NB_E_ARG, "nb_render_view: null argument");
   if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_render_view: nothing uploaded");
@@ -575,30 +605,38 @@ int nb_render_view(nb_ctx* c, float* xyzs) {
     for (int k = 0; k < 4; ++k) xyzs[4 * i + k] = h[4 * s + k];
   }
   return check_flags(c);
+  NB_CATCH("nb_render_view")
 }
 
 int nb_get_order(nb_ctx* c, int32_t* order) {
+  NB_TRY
   if (!c || !order) return nb_fail(NB_E_ARG, "nb_get_order: null argument");
   for (int64_t k = 0; k < c->n; ++k) order[k] = c->order.empty() ? (int32_t)k : c->order[k];
   return NB_OK;
+  NB_CATCH("nb_get_order")
 }
 
 int nb_set_timing(nb_ctx* c, int32_t enable) {
+  NB_TRY
   NB_API_SHARED;
   if (!c) return nb_fail(NB_E_ARG, "nb_set_timing: null context");
   NB_HIP(hipStreamSynchronize(c->stream));
   c->timing = enable != 0;
   c->tim = nb_timing{};
   return NB_OK;
+  NB_CATCH("nb_set_timing")
 }
 
 int nb_get_timing(nb_ctx* c, nb_timing* out) {
+  NB_TRY
   if (!c || !out) return nb_fail(NB_E_ARG, "nb_get_timing: null argument");
   *out = c->tim;
   return NB_OK;
+  NB_CATCH("nb_get_timing")
 }
 
 int nb_count_interactions(nb_ctx* c, int64_t* interactions) {
+  NB_TRY
   NB_API_SHARED;
   if (!c || !interactions) return nb_fail(NB_E_ARG, "nb_count_interactions: null argument");
   if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_count_interactions: nothing uploaded");
@@ -614,9 +652,11 @@ int nb_count_interactions(nb_ctx* c, int64_t* interactions) {
   *interactions = (int64_t)h;
   c->tim.interactions = (int64_t)h;
   return check_flags(c);
+  NB_CATCH("nb_count_interactions")
 }
 
 int nb_device_info(char* buf, int32_t cap) {
+  NB_TRY
   if (!buf || cap < 1) return nb_fail(NB_E_ARG, "nb_device_info: bad buffer");
   int dev = 0;
   hipDeviceProp_t prop;
@@ -628,6 +668,7 @@ int nb_device_info(char* buf, int32_t cap) {
                 prop.multiProcessorCount, prop.clockRate / 1000.0,
                 prop.totalGlobalMem / 1073741824.0);
   return NB_OK;
+  NB_CATCH("nb_device_info")
 }
 
 }  // extern "C"

@@ -4,7 +4,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <exception>
 #include <mutex>
+#include <new>
 #include <shared_mutex>
 #include <string>
 #include <vector>
@@ -28,6 +30,22 @@ int nb_fail(int code, const std::string& msg);
 std::shared_mutex& nb_capture_mutex();
 #define NB_API_SHARED std::shared_lock<std::shared_mutex> nb_api_guard_(nb_capture_mutex())
 #define NB_API_CAPTURE std::unique_lock<std::shared_mutex> nb_capture_guard_(nb_capture_mutex())
+
+// Every int-returning C entry point runs its body inside NB_TRY ... NB_CATCH so
+// that no C++ exception (a failed host allocation, a thread that cannot start)
+// crosses the C ABI: it becomes a status code and an nb_last_error() message.
+#define NB_TRY try {
+#define NB_CATCH(fn)                                                                  \
+  }                                                                                   \
+  catch (const std::bad_alloc&) {                                                     \
+    return nb_fail(NB_E_OOM, fn ": host allocation failed");                          \
+  }                                                                                   \
+  catch (const std::exception& nb_ex_) {                                              \
+    return nb_fail(NB_E_HIP, std::string(fn ": ") + nb_ex_.what());                   \
+  }                                                                                   \
+  catch (...) {                                                                       \
+    return nb_fail(NB_E_HIP, fn ": unexpected exception");                            \
+  }
 
 // Padding granule of the FP32 body array: the direct kernel's i-blocks hold
 // 64 lanes x up to 4 bodies, and its j-range is split over up to 4 waves in

@@ -3,6 +3,7 @@
 #include <fstream>
 #include <iomanip>
 #include <limits>
+#include <new>
 #include <sstream>
 #include <string>
 
@@ -32,8 +33,10 @@ bool parse_triplet(const std::string& line, float out[3]) {
 
 }  // namespace
 
-extern "C" int64_t nb_ic_read_file(const char* path, int64_t cap, float* pos4, float* vel4,
-                                   float* acc4, float* mass) {
+namespace {
+
+int64_t read_file(const char* path, int64_t cap, float* pos4, float* vel4, float* acc4,
+                  float* mass) {
   if (!path) return NB_E_ARG;
   std::ifstream file(path);
   if (!file) return NB_E_ARG;
@@ -81,8 +84,8 @@ extern "C" int64_t nb_ic_read_file(const char* path, int64_t cap, float* pos4, f
   return count;
 }
 
-extern "C" int nb_ic_write_file(const char* path, int64_t n, const float* pos4, const float* vel4,
-                                const float* acc4, const float* mass) {
+int write_file(const char* path, int64_t n, const float* pos4, const float* vel4,
+               const float* acc4, const float* mass) {
   if (!path || n < 0 || !pos4 || !vel4 || !mass) return NB_E_ARG;
   std::ofstream os(path);
   if (!os) return NB_E_ARG;
@@ -103,4 +106,30 @@ extern "C" int nb_ic_write_file(const char* path, int64_t n, const float* pos4, 
     os << "Mass: " << mass[i] << "\n";
   }
   return os.good() ? NB_OK : NB_E_ARG;
+}
+
+}  // namespace
+
+// The C entry points: no C++ exception crosses them (a failed host allocation
+// while reading or writing becomes NB_E_OOM).
+extern "C" int64_t nb_ic_read_file(const char* path, int64_t cap, float* pos4, float* vel4,
+                                   float* acc4, float* mass) {
+  try {
+    return read_file(path, cap, pos4, vel4, acc4, mass);
+  } catch (const std::bad_alloc&) {
+    return NB_E_OOM;
+  } catch (...) {
+    return NB_E_ARG;
+  }
+}
+
+extern "C" int nb_ic_write_file(const char* path, int64_t n, const float* pos4, const float* vel4,
+                                const float* acc4, const float* mass) {
+  try {
+    return write_file(path, n, pos4, vel4, acc4, mass);
+  } catch (const std::bad_alloc&) {
+    return NB_E_OOM;
+  } catch (...) {
+    return NB_E_ARG;
+  }
 }

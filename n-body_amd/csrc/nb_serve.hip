@@ -279,6 +279,7 @@ void free_server(nb_server* s) {
 extern "C" {
 
 int nb_server_create(nb_server** out, const nb_server_config* cfg) {
+  NB_TRY
   if (!out || !cfg) return nb_fail(NB_E_ARG, "nb_server_create: null argument");
   *out = nullptr;
   nb_server* s = new (std::nothrow) nb_server();
@@ -318,9 +319,15 @@ int nb_server_create(nb_server** out, const nb_server_config* cfg) {
   }
   s->budget_total = s->bytes_left = budget - pool_bytes;
   s->pool_left = s->cfg.direct_pool;
-  for (int t = 0; t < s->cfg.ic_threads; ++t) s->workers.emplace_back([s] { s->worker(); });
+  try {
+    for (int t = 0; t < s->cfg.ic_threads; ++t) s->workers.emplace_back([s] { s->worker(); });
+  } catch (...) {  // stop and join the workers that did start
+    free_server(s);
+    return nb_fail(NB_E_HIP, "nb_server_create: cannot start the worker threads");
+  }
   *out = s;
   return NB_OK;
+  NB_CATCH("nb_server_create")
 }
 
 void nb_server_destroy(nb_server* s) { free_server(s); }
@@ -328,6 +335,7 @@ void nb_server_destroy(nb_server* s) { free_server(s); }
 double nb_server_now(nb_server* s) { return s ? s->now() : 0.0; }
 
 int nb_server_submit(nb_server* s, const nb_job* job, int64_t* id) {
+  NB_TRY
   if (!s || !job || !id) return nb_fail(NB_E_ARG, "nb_server_submit: null argument");
   if (job->n < 1 || job->n >= (int64_t(1) << 27) || job->steps < 0 ||
       (job->ic != NB_IC_PLUMMER && job->ic != NB_IC_BALL) || !(job->eps2 >= 0.f))
@@ -349,9 +357,11 @@ int nb_server_submit(nb_server* s, const nb_job* job, int64_t* id) {
   s->queue.push_back(jid);
   *id = jid;
   return NB_OK;
+  NB_CATCH("nb_server_submit")
 }
 
 int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
+  NB_TRY
   if (!s) return nb_fail(NB_E_ARG, "nb_server_tick: null server");
   int finished = 0;
   // 0. Barnes-Hut jobs whose worker has copied them out are done
@@ -496,18 +506,22 @@ int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
   if (running_out) *running_out = (int32_t)s->running.size();
   if (finished_out) *finished_out = finished;
   return NB_OK;
+  NB_CATCH("nb_server_tick")
 }
 
 int nb_server_poll(nb_server* s, int64_t* ids, int32_t cap, int32_t* count) {
+  NB_TRY
   if (!s || !count || (cap > 0 && !ids)) return nb_fail(NB_E_ARG, "nb_server_poll: bad argument");
   const int32_t k = (int32_t)std::min<size_t>(cap > 0 ? (size_t)cap : 0, s->retired.size());
   std::copy(s->retired.begin(), s->retired.begin() + k, ids);
   s->retired.erase(s->retired.begin(), s->retired.begin() + k);
   *count = k;
   return NB_OK;
+  NB_CATCH("nb_server_poll")
 }
 
 int nb_server_status(nb_server* s, int64_t id, nb_job_status* out) {
+  NB_TRY
   if (!s || !out) return nb_fail(NB_E_ARG, "nb_server_status: null argument");
   auto it = s->jobs.find(id);
   if (it == s->jobs.end()) return nb_fail(NB_E_ARG, "nb_server_status: unknown job");
@@ -520,9 +534,11 @@ int nb_server_status(nb_server* s, int64_t id, nb_job_status* out) {
   out->t_admit = j.t_admit;
   out->t_done = j.t_done;
   return NB_OK;
+  NB_CATCH("nb_server_status")
 }
 
 int nb_server_result(nb_server* s, int64_t id, double* pos3, double* vel3) {
+  NB_TRY
   if (!s) return nb_fail(NB_E_ARG, "nb_server_result: null server");
   auto it = s->jobs.find(id);
   if (it == s->jobs.end()) return nb_fail(NB_E_ARG, "nb_server_result: unknown job");
@@ -531,9 +547,11 @@ int nb_server_result(nb_server* s, int64_t id, double* pos3, double* vel3) {
   if (pos3) std::copy(j.pos.begin(), j.pos.end(), pos3);
   if (vel3) std::copy(j.vel.begin(), j.vel.end(), vel3);
   return NB_OK;
+  NB_CATCH("nb_server_result")
 }
 
 int nb_server_forget(nb_server* s, int64_t id) {
+  NB_TRY
   if (!s) return nb_fail(NB_E_ARG, "nb_server_forget: null server");
   auto it = s->jobs.find(id);
   if (it == s->jobs.end()) return nb_fail(NB_E_ARG, "nb_server_forget: unknown job");
@@ -542,6 +560,7 @@ int nb_server_forget(nb_server* s, int64_t id) {
   std::lock_guard<std::mutex> lk(s->mu);
   s->jobs.erase(it);
   return NB_OK;
+  NB_CATCH("nb_server_forget")
 }
 
 }  // extern "C"

@@ -84,6 +84,7 @@ bool nb_tuned_shape(int64_t n, int* bpl, int* jsplit, int* leaf) {
 }
 
 extern "C" int nb_autotune(int64_t n, int32_t what, int32_t reps, nb_tuning* out) {
+  NB_TRY
   if (n < 1 || n >= (int64_t(1) << 27)) return nb_fail(NB_E_ARG, "nb_autotune: n out of range");
   if (reps < 1) reps = 3;
   const Input in = plummer(n);
@@ -144,9 +145,11 @@ extern "C" int nb_autotune(int64_t n, int32_t what, int32_t reps, nb_tuning* out
   }
   if (out) *out = best;
   return NB_OK;
+  NB_CATCH("nb_autotune")
 }
 
 extern "C" int nb_get_tuning(int64_t n, nb_tuning* out) {
+  NB_TRY
   if (!out || n < 1) return nb_fail(NB_E_ARG, "nb_get_tuning: bad argument");
   nb_config cfg{};
   cfg.n = n;
@@ -159,4 +162,5 @@ extern "C" int nb_get_tuning(int64_t n, nb_tuning* out) {
   out->j_split = js;
   out->leaf_size = leaf;
   return NB_OK;
+  NB_CATCH("nb_get_tuning")
 }
