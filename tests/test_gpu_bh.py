@@ -210,7 +210,7 @@ def test_largest_tested_size_2_24(nb, ora):
     """N = 2^24 Plummer sphere (four times the C3 size): Morton codes, permutation
     and octree topology bit-exact against the CPU build, the root mass equal to
     the total mass, and theta=0.5 forces within the C3 accuracy on seeded targets.
-    Exercises the multi-block scans and sort at 4096 sort tiles."""
+    Exercises the multi-block scans and a sort of 8192 tiles."""
     n = 1 << 24
     pos, vel, m = nb.ic_plummer(2424, n, 50.0, 1e4)
     print(f"N=2^24: uploading {n} bodies", flush=True)
