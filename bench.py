@@ -13,7 +13,9 @@ Also reported on the same line:
   cpu_baseline  the oracle's v2 restatement (OpenMP, the reference's flags) on a
                 bounded sample of the same workload on this host;
   bh_4m         Barnes-Hut step time on two colliding exponential discs,
-                N = 2^22, theta = 0.5 (skip with --no-bh); its roofline object
+                N = 2^22, theta = 0.5 (skip with --no-bh), with the force error on
+                seeded targets and the FP64 energy drift over the run (both
+                outside the timed steps); its roofline object
                 prices the traversal kernel at 20 flop per interaction counted on
                 the first timed step, over the mean traversal launch time;
   batch_c4      512 independent Plummer clusters (N_c log-uniform in
@@ -190,6 +192,7 @@ def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0, accuracy: boo
     with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH) as sim:  # tuned leaf size
         sim.upload(pos, vel, m)
         inter = sim.count_interactions()
+        e0 = sum(sim.energy()[:2]) if accuracy else None  # FP64 direct potential, untimed
         sim.step(warmup)
         sim.sync()
         sim.set_timing(True)
@@ -204,8 +207,12 @@ def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0, accuracy: boo
             pf, _, _ = sim.download_f64()
             f = sim.forces_f64()
             tg = np.random.default_rng(7).choice(n, 64, replace=False)
-            acc = force_errors(pf, m, f, tg)
-            log(f"bh: force error vs FP64 p50 {acc['p50']:.2e} p99 {acc['p99']:.2e}")
+            fe = force_errors(pf, m, f, tg)
+            e1 = sum(sim.energy()[:2])
+            acc = {"force_error_vs_fp64": fe, "energy_drift": abs(e1 - e0) / abs(e0),
+                   "energy_steps": warmup + steps}
+            log(f"bh: force error vs FP64 p50 {fe['p50']:.2e} p99 {fe['p99']:.2e}; "
+                f"energy drift {acc['energy_drift']:.2e} over {warmup + steps} steps")
     ms = 1e3 * wall / steps
     cpu = None
     if cpu_budget > 0:
@@ -222,7 +229,7 @@ def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0, accuracy: boo
         "interactions_first_step": inter,
         "interactions_per_s": inter / (t["force_ms"] / max(1, t["force_calls"]) / 1e3),
         "roofline": valu_roofline("k_traverse_list", inter, t["force_ms"] / max(1, t["force_calls"])),
-        "force_error_vs_fp64": acc,
+        "accuracy": acc,
         "cpu_baseline": cpu,
     }
 
