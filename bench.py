@@ -464,7 +464,10 @@ def main():
     ap.add_argument("--bh-steps", type=int, default=20)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-steps", type=int, default=20)
-    ap.add_argument("--batch-energy", action="store_true", help="per-cluster FP64 energy drift")
+    ap.add_argument("--batch-energy", action="store_true",
+                    help="kept for old command lines: the per-cluster FP64 energy drift is always reported")
+    ap.add_argument("--no-batch-energy", action="store_true",
+                    help="skip the per-cluster FP64 energies (outside the timed steps)")
     ap.add_argument("--serve", type=float, default=20.0,
                     help="seconds of the serving stream per mode (0 skips it)")
     ap.add_argument("--serve-rate", type=float, default=50.0)
@@ -492,7 +495,7 @@ def main():
         print(json.dumps(bench_bh(nb, args.bh_steps, 2, 0.0, not args.no_accuracy)), flush=True)
         return
     if args.only == "batch":
-        print(json.dumps(bench_batch(nb, args.batch_steps, 2, args.batch_energy, not args.no_tune)),
+        print(json.dumps(bench_batch(nb, args.batch_steps, 2, not args.no_batch_energy, not args.no_tune)),
               flush=True)
         return
     if args.only == "direct":
@@ -580,7 +583,7 @@ def main():
         log(f"bh: {result['bh_4m']['ms_per_step']:.2f} ms/step")
     if not args.no_batch:
         log("batch: C4 ensemble")
-        result["batch_c4"] = bench_batch(nb, args.batch_steps, 2, args.batch_energy, not args.no_tune)
+        result["batch_c4"] = bench_batch(nb, args.batch_steps, 2, not args.no_batch_energy, not args.no_tune)
         log(f"batch: {result['batch_c4']['ms_per_step']:.2f} ms/step")
     if args.serve > 0:
         result["serve"] = bench_serve(nb, args.serve_rate, args.serve, args.serve)
