@@ -481,7 +481,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this engine has no multi-GPU path "
-              "(see DESIGN.md for how it would shard); run with --gpus 1", file=sys.stderr)
+              "(the direct sum would shard its targets over GPUs with one all-gather of "
+              "the sources per step); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_probe > 0:
