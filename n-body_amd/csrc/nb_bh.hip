@@ -634,11 +634,15 @@ __device__ __forceinline__ void digit_bounds(const uint32_t* __restrict__ codes,
   for (int d = 0; d < 7; ++d) bound[d + 1] = lo[d];
 }
 
+// Counts the non-empty children of every internal node of level l and keeps
+// their boundaries (bound[1..7]; bound[0] and bound[8] are the node's own range)
+// for k_tree_emit, which would otherwise repeat the seven binary searches.
 __global__ __launch_bounds__(256) void k_tree_count(const uint32_t* __restrict__ codes, int n,
                                                     const int4* __restrict__ link,
                                                     const int* __restrict__ level_off, int l,
                                                     int leaf, int cap,
-                                                    int* __restrict__ child_cnt) {
+                                                    int* __restrict__ child_cnt,
+                                                    int* __restrict__ child_bounds) {
   const int lo = level_off[l];
   const int hi = min(level_off[l + 1], lo + cap);  // cap = launch bound of this level
   const int shift = 27 - 3 * l;
@@ -651,6 +655,8 @@ __global__ __launch_bounds__(256) void k_tree_count(const uint32_t* __restrict__
       digit_bounds(codes, nd.z, nd.z + cnt, shift, bound);
 #pragma unroll
       for (int d = 0; d < 8; ++d) nc += bound[d + 1] > bound[d] ? 1 : 0;
+#pragma unroll
+      for (int d = 1; d < 8; ++d) child_bounds[8 * (k - lo) + d] = bound[d];
     }
     child_cnt[k - lo] = nc;
   }
@@ -687,13 +693,13 @@ __global__ __launch_bounds__(256) void k_tree_emit(const uint32_t* __restrict__ 
                                                    const int* __restrict__ level_off, int l,
                                                    int leaf, int cap,
                                                    const int* __restrict__ child_off,
+                                                   const int* __restrict__ child_bounds,
                                                    int64_t max_nodes, int* __restrict__ flags,
                                                    int2* __restrict__ waves,
                                                    int* __restrict__ wctl) {
   const int lo = level_off[l];
   const int hi = min(level_off[l + 1], lo + cap);
   const int next_lo = level_off[l + 1];
-  const int shift = 27 - 3 * l;
   const int lane = threadIdx.x & 63;
   // Rounds are uniform across the block, so every lane of a wave reaches the
   // wave-wide range allocation below.
@@ -708,8 +714,11 @@ __global__ __launch_bounds__(256) void k_tree_emit(const uint32_t* __restrict__ 
       internal = cnt > leaf && nd.z >= 0 && nd.z + cnt <= n;
     }
     if (internal) {
-      int bound[9];
-      digit_bounds(codes, nd.z, nd.z + link_count(nd.w), shift, bound);
+      int bound[9];  // as k_tree_count found them
+      bound[0] = nd.z;
+      bound[8] = nd.z + link_count(nd.w);
+#pragma unroll
+      for (int d = 1; d < 8; ++d) bound[d] = child_bounds[8 * (k - lo) + d];
 #pragma unroll
       for (int d = 0; d < 8; ++d)
         if (bound[d + 1] > bound[d]) {
@@ -1358,6 +1367,7 @@ int nb_bh_alloc(nb_ctx* c) {
       A((void**)&b.node_link, sizeof(int4) * bound) ||
       A((void**)&b.level_off, sizeof(int) * (kMaxLevel + 2)) ||
       A((void**)&b.child_cnt, sizeof(int) * n) || A((void**)&b.child_off, sizeof(int) * n) ||
+      A((void**)&b.child_bounds, sizeof(int) * 8 * n) ||
       A((void**)&b.counters, sizeof(int) * 16) ||
       A((void**)&b.top_start, sizeof(int) * kTopEntries) ||
       A((void**)&b.waves, sizeof(int2) * n) ||
@@ -1375,7 +1385,7 @@ void nb_bh_free(nb_ctx* c) {
   void* ps[] = {b.bbox_part, b.geom,     b.keys[0],  b.keys[1],   b.vals[0],    b.vals[1],
                 b.codes,     b.digit_hist, b.pms,    b.prefix,    b.scan_part,  b.node_com,
                 b.node_link, b.level_off, b.child_cnt, b.child_off, b.iscan_part, b.counters, b.waves,
-                b.inter,     b.top_start};
+                b.inter,     b.top_start, b.child_bounds};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   b = nb_bh_state{};
@@ -1454,13 +1464,13 @@ hipError_t nb_bh_build(nb_ctx* c) {
     const int bound = level_bound(n, l);
     const int grid = std::max(1, std::min((bound + 255) / 256, 4096));
     hipLaunchKernelGGL(k_tree_count, dim3(grid), dim3(256), 0, s, codes_sorted, n, b.node_link,
-                       b.level_off, l, leaf, bound, b.child_cnt);
+                       b.level_off, l, leaf, bound, b.child_cnt, b.child_bounds);
     hipError_t e = iscan(b.child_cnt, b.child_off, b.iscan_part, b.level_off, l, 0, bound,
                          b.level_off, s);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_tree_emit, dim3(grid), dim3(256), 0, s, codes_sorted, n, b.node_link,
-                       b.level_off, l, leaf, bound, b.child_off, b.max_nodes, c->flags, b.waves,
-                       b.counters);
+                       b.level_off, l, leaf, bound, b.child_off, b.child_bounds, b.max_nodes,
+                       c->flags, b.waves, b.counters);
   }
   // 7. centre of mass
   const int cgrid = (int)std::max<int64_t>(1, std::min<int64_t>((b.max_nodes + 255) / 256, 8192));

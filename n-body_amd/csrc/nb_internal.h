@@ -78,6 +78,7 @@ struct nb_bh_state {
   int* level_off = nullptr;     // [kMaxLevel + 2] node index where each level begins
   int* child_cnt = nullptr;     // per node of the current level: non-empty children
   int* child_off = nullptr;     // exclusive scan of child_cnt
+  int* child_bounds = nullptr;  // per node of the current level: its 7 inner child boundaries [8 n]
   int* iscan_part = nullptr;    // int scan block partials
   int* counters = nullptr;      // [0] wave ranges, [1] traversal ticket
   int2* waves = nullptr;        // wave ranges (first sorted body, count <= 64)

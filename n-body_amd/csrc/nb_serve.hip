@@ -74,7 +74,7 @@ int64_t tree_bound(int64_t n) {
 // Device bytes of a Barnes-Hut context of n bodies (nb_create + nb_bh_alloc).
 int64_t bh_bytes(int64_t n) {
   const int64_t np = (n + 255) / 256 * 256;
-  return np * (16 + 4) + n * (4 * 24 + 4 * 5 + 16 + 32 + 8) + 32 * tree_bound(n) + (4 << 20);
+  return np * (16 + 4) + n * (4 * 24 + 4 * 5 + 16 + 32 + 8 + 32) + 32 * tree_bound(n) + (4 << 20);
 }
 
 int64_t padded(int64_t n) { return (n + kPadBodies - 1) / kPadBodies * kPadBodies; }
