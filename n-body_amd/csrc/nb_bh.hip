@@ -1062,9 +1062,11 @@ constexpr uint32_t kCellBit = 0x80000000u;    // list entry refers to a cell, no
 template <bool kCount, bool kGuard>
 __device__ __forceinline__ void list_flush(const uint32_t* list, int lc, float4* stage, int lane,
                                            int slice, int log2S, const float4* __restrict__ pms,
-                                           const float4* __restrict__ com, float4 me, float eps2,
+                                           const float4* __restrict__ com, float4 me, float eps2_s,
                                            float& ax, float& ay, float& az,
                                            unsigned long long& cnt) {
+  float eps2 = eps2_s;  // a VGPR copy: all-VGPR operand forms issue faster
+  asm volatile("" : "+v"(eps2));
   const int S = 1 << log2S;
   const int per = 64 >> log2S;  // entries per slice
   const int put = (lane & (S - 1)) * per + (lane >> log2S);  // slice-major slot of entry `lane`

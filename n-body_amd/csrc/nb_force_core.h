@@ -32,7 +32,11 @@ struct Targets {
 // Accumulate the sources [j0, j1) (wave-uniform, j1 - j0 a multiple of 16).
 template <int B, bool kGuard>
 __device__ __forceinline__ void accumulate(const float4* __restrict__ pm, int j0, int j1,
-                                           float eps2, Targets<B>& t) {
+                                           float eps2_s, Targets<B>& t) {
+  // eps2 is uniform and would stay an SGPR operand; a VGPR copy (the empty asm
+  // hides its uniformity) turns the r^2 seed v_fma into an all-VGPR form.
+  float eps2 = eps2_s;
+  asm volatile("" : "+v"(eps2));
   for (int jc = j0; jc < j1; jc += kChunk) {
     const int je = min(jc + kChunk, j1);
     float px[B], py[B], pz[B];
