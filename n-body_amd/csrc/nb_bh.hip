@@ -1042,11 +1042,14 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ pms
 // gathers the sources 64 at a time into LDS and streams them through the
 // direct-sum inner loop (LDS broadcast reads, 13 VALU per interaction).
 constexpr int kListWaves = 4;
-constexpr int kListBlocksPerCU = 5;  // LDS (28.8 KB per block) and 96 VGPRs allow five
+// LDS (26.7 KB per block) and 74 VGPRs allow six blocks per CU. A 384-entry
+// list (instead of 512) is what makes the sixth fit; it measured about 1% faster
+// on the two-disc 4M system.
+constexpr int kListBlocksPerCU = 6;
 // The deepest stack seen on the two-disc 4M system (400 sampled ranges, leaf 8)
 // is 73 entries; overflow is detected and reported (NB_E_TREE_OVERFLOW).
 constexpr int kStackCap = 512;
-constexpr int kListCap = 512;                 // source indices per wave
+constexpr int kListCap = 384;                 // source indices per wave
 constexpr uint32_t kCellBit = 0x80000000u;    // list entry refers to a cell, not a body
 
 // Stream the list through the direct-sum inner loop: 64 entries at a time,
@@ -1071,15 +1074,22 @@ __device__ __forceinline__ void list_flush(const uint32_t* list, int lc, float4*
   const int per = 64 >> log2S;  // entries per slice
   const int put = (lane & (S - 1)) * per + (lane >> log2S);  // slice-major slot of entry `lane`
   const float4* mine = stage + slice * per;
+  // lane's source of the chunk starting at e (a zero-mass pad past the list)
+  auto gather = [&](int e) {
+    float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (e + lane < lc) {
+      const uint32_t id = list[e + lane];
+      g = (id & kCellBit) ? com[id & ~kCellBit] : pms[id];
+    }
+    return g;
+  };
+  float4 q = gather(0);
   for (int e = 0; e < lc; e += 64) {
     const int m = min(64, lc - e);
-    float4 q = make_float4(0.f, 0.f, 0.f, 0.f);  // zero-mass pad
-    if (lane < m) {
-      const uint32_t id = list[e + lane];
-      q = (id & kCellBit) ? com[id & ~kCellBit] : pms[id];
-    }
     stage[put] = q;
     __builtin_amdgcn_wave_barrier();
+    // the next chunk's gather is in flight while this chunk is evaluated
+    if (e + 64 < lc) q = gather(e + 64);
     // T = ceil(m / S) rounded up to 4 entries per slice; S <= 16 divides 64, so
     // 64 / S is a multiple of 4 and T <= 64 / S: every read stays inside this
     // slice's block; entries at or past m are zero-mass pads.
