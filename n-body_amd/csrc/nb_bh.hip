@@ -1231,9 +1231,11 @@ __device__ void traverse_range(const WaveLds& L, int lane, int P, float4 me, int
 }
 
 // Persistent: every wave takes wave ranges from a ticket until none is left,
-// so the cost of uneven ranges balances dynamically.
+// so the cost of uneven ranges balances dynamically. The counting variant
+// (diagnostics only) needs more registers and is allowed five blocks per CU;
+// its surplus blocks start when others retire and take what tickets are left.
 template <bool kCount, bool kGuard>
-__global__ __launch_bounds__(64 * kListWaves, kListBlocksPerCU) void k_traverse_list(
+__global__ __launch_bounds__(64 * kListWaves, kCount ? 5 : kListBlocksPerCU) void k_traverse_list(
     const float4* __restrict__ pms, const uint32_t* __restrict__ perm, int n,
     const float4* __restrict__ com, const int4* __restrict__ link,
     const int* __restrict__ level_off, int64_t max_nodes, const float* __restrict__ geom,
