@@ -68,6 +68,12 @@ int nb_ic_cube_surface(uint64_t seed, int64_t n, double world_size, double* pos3
  * the seed, as the reference draws it once per system. */
 int nb_ic_sphere_surface(uint64_t seed, int64_t n, double world_size, int pattern, double* pos3,
                          double* vel3, float* mass);
+/* Ball (-i 5, ParticleSystemBall.cpp:10-92): uniform in the ball of radius
+ * R = sqrt(3) w / 3 about (w/2, w/2, w/2) (rejection from its bounding cube),
+ * then each coordinate nudged by U(-1, 1) * 0.05 R; velocity components
+ * U(-1, 1) * 0.001 sqrt(3) w; masses (w^3/n) * (0.5 + 4.5 u^2). */
+int nb_ic_ball_world(uint64_t seed, int64_t n, double world_size, double* pos3, double* vel3,
+                     float* mass);
 /* Disc galaxy (-i 2, the reference's default, ParticleSystemGalaxyInitializer.cpp:
  * 12-210), centred at (w/2, w/2, w/2) in the z = w/2 plane. style 0 (lines 46-91):
  * radius U(0, w), angle 2 pi N(0, 50 pi), height U(0.1, w/5), tangential speed

@@ -244,6 +244,32 @@ int nb_ic_sphere_surface(uint64_t seed, int64_t n, double world_size, int patter
   return 0;
 }
 
+int nb_ic_ball_world(uint64_t seed, int64_t n, double world_size, double* pos3, double* vel3,
+                     float* mass) {
+  if (n < 1 || !pos3 || !vel3 || !mass || !(world_size > 0)) return -1;
+  // ParticleSystemBall.cpp:19-89: a uniform ball by rejection from its bounding
+  // cube, each body nudged by up to 5% of the radius per axis, small uniform
+  // velocities, masses skewed towards the light end by u^2.
+  Rng r(seed);
+  const double w = world_size;
+  const double diag = std::sqrt(3.0) * w;
+  const double radius = diag / 3.0, c = 0.5 * w, nudge = 0.05 * radius;
+  const double vscale = 0.001 * diag;
+  const double avg = w * w * w / (double)n;
+  for (int64_t i = 0; i < n; ++i) {
+    double d[3], r2;
+    do {
+      for (int k = 0; k < 3; ++k) d[k] = r.uniform(-1.0, 1.0) * radius;
+      r2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    } while (r2 > radius * radius);
+    for (int k = 0; k < 3; ++k) pos3[3 * i + k] = c + d[k] + r.uniform(-1.0, 1.0) * nudge;
+    for (int k = 0; k < 3; ++k) vel3[3 * i + k] = r.uniform(-1.0, 1.0) * vscale;
+    const double u = r.uniform();
+    mass[i] = (float)(avg * (0.5 + 4.5 * u * u));
+  }
+  return 0;
+}
+
 int nb_ic_galaxy(uint64_t seed, int64_t n, double world_size, int style, double* pos3,
                  double* vel3, float* mass) {
   if (n < 1 || !pos3 || !vel3 || !mass || !(world_size > 0) || style < -1 || style > 1)

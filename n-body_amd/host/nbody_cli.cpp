@@ -8,7 +8,7 @@
 //   -n N     number of bodies          -t dt     time step
 //   -i 1..6   the reference's initializers in its world cube of edge 5
 //            (main.cpp:29): 1 cube, 2 disc galaxy (default), 3 Lagrange triangle,
-//            4 sphere surface, 5 uniform ball (R=100, m=1), 6 cube surface
+//            4 sphere surface, 5 ball, 6 cube surface
 //   -i 8|9   Plummer sphere (a=50, M=1e4), two colliding discs
 //            every random initializer is seeded (--seed)
 //   -s eps2  squared softening         -f path   particle file (reference format)
@@ -59,7 +59,7 @@ std::vector<Particle> generate(int init, int64_t n, uint64_t seed, float eps2) {
     case 1: rc = nb_ic_cube(seed, n, kWorld, p.data(), v.data(), m.data()); break;
     case 2: rc = nb_ic_galaxy(seed, n, kWorld, -1, p.data(), v.data(), m.data()); break;
     case 4: rc = nb_ic_sphere_surface(seed, n, kWorld, -1, p.data(), v.data(), m.data()); break;
-    case 5: rc = nb_ic_ball(seed, n, 100.0, 1.0, p.data(), v.data(), m.data()); break;
+    case 5: rc = nb_ic_ball_world(seed, n, kWorld, p.data(), v.data(), m.data()); break;
     case 6: rc = nb_ic_cube_surface(seed, n, kWorld, p.data(), v.data(), m.data()); break;
     case 8: rc = nb_ic_plummer(seed, n, 50.0, 1e4, 10.0, 1.0, p.data(), v.data(), m.data()); break;
     case 9:

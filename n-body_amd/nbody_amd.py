@@ -235,6 +235,7 @@ _SIGS = {
     "nb_ic_lagrange": (C.c_int, [C.c_double, _P, _P, _P]),
     "nb_ic_cube": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, _P, _P, _P]),
     "nb_ic_cube_surface": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, _P, _P, _P]),
+    "nb_ic_ball_world": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, _P, _P, _P]),
     "nb_ic_sphere_surface": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, C.c_int, _P, _P, _P]),
     "nb_ic_galaxy": (C.c_int, [C.c_uint64, C.c_int64, C.c_double, C.c_int, _P, _P, _P]),
     "nb_ic_read_file": (C.c_int64, [C.c_char_p, C.c_int64, _P, _P, _P, _P]),
@@ -345,6 +346,11 @@ def ic_cube(seed: int, n: int, world_size: float = 5.0):
 def ic_cube_surface(seed: int, n: int, world_size: float = 5.0):
     """Reference -i 6 (ParticleSystemCubeSurface.cpp:8-49), seeded."""
     return _ic_world(load().nb_ic_cube_surface, seed, n, world_size)
+
+
+def ic_ball_world(seed: int, n: int, world_size: float = 5.0):
+    """Reference -i 5 (ParticleSystemBall.cpp:10-92), seeded."""
+    return _ic_world(load().nb_ic_ball_world, seed, n, world_size)
 
 
 def ic_sphere_surface(seed: int, n: int, world_size: float = 5.0, pattern: int = -1):

@@ -21,11 +21,11 @@ def run_cli(*args):
 
 def test_cli_direct_matches_binding(nb, tmp_path):
     out = tmp_path / "final.txt"
-    txt = run_cli("-v", 4, "-i", 5, "-n", 2048, "-t", 0.0035, "-s", 40, "--steps", 10, "--seed", 3,
+    txt = run_cli("-v", 4, "-i", 8, "-n", 2048, "-t", 0.0035, "-s", 40, "--steps", 10, "--seed", 3,
                   "--out", out, "--energy")
     assert "interactions/s" in txt and "drift" in txt
     p, v, a, m = nb.read_particle_file(out)
-    pos, vel, mass = nb.ic_ball(3, 2048, 100.0, 1.0)
+    pos, vel, mass = nb.ic_plummer(3, 2048, 50.0, 1e4)
     with nb.Simulation(2048, 40.0, 0.0035) as sim:
         sim.upload(nb.to_vec4(pos.astype(np.float32)), nb.to_vec4(vel.astype(np.float32)), mass)
         sim.step(10)
@@ -36,9 +36,9 @@ def test_cli_direct_matches_binding(nb, tmp_path):
 
 
 @pytest.mark.parametrize("init,gen", [(1, "ic_cube"), (2, "ic_galaxy"), (4, "ic_sphere_surface"),
-                                      (6, "ic_cube_surface")])
+                                      (5, "ic_ball_world"), (6, "ic_cube_surface")])
 def test_cli_reference_initializers(nb, tmp_path, init, gen):
-    """-i 1, 2, 4, 6 with the reference's defaults (dt 0.001, eps2 0.0935, world
+    """-i 1, 2, 4, 5, 6 with the reference's defaults (dt 0.001, eps2 0.0935, world
     edge 5): the direct-sum run matches the binding on the same seeded bodies bit
     for bit, and Barnes-Hut runs the same start."""
     n, seed = 3000, 9

@@ -1,8 +1,8 @@
-"""The reference's visual initializers (-i 1, 2, 4, 6), seeded: geometry, velocity
+"""The reference's visual initializers (-i 1, 2, 4, 5, 6), seeded: geometry, velocity
 and mass rules of ParticleSystemCubeInitializer.cpp:8-29,
-ParticleSystemGalaxyInitializer.cpp:12-210, ParticleSystemSphere.cpp:20-133 and
-ParticleSystemCubeSurface.cpp:8-49 at the reference's world edge w = 5
-(main.cpp:29). Host-only generators: no GPU is needed.
+ParticleSystemGalaxyInitializer.cpp:12-210, ParticleSystemSphere.cpp:20-133,
+ParticleSystemBall.cpp:10-92 and ParticleSystemCubeSurface.cpp:8-49 at the
+reference's world edge w = 5 (main.cpp:29). Host-only generators: no GPU is needed.
 
 The reference draws from an unseeded std::mt19937, so no byte of its output can
 be reproduced; each test checks a property the reference's construction fixes
@@ -19,7 +19,8 @@ def zscore(sample_mean, expected, var, n):
     return abs(sample_mean - expected) / np.sqrt(var / n)
 
 
-@pytest.mark.parametrize("name", ["ic_cube", "ic_cube_surface", "ic_sphere_surface", "ic_galaxy"])
+@pytest.mark.parametrize("name", ["ic_cube", "ic_cube_surface", "ic_sphere_surface", "ic_galaxy",
+                                  "ic_ball_world"])
 def test_seeded_and_deterministic(nb, name):
     f = getattr(nb, name)
     a, b, c = f(11, 4000), f(11, 4000), f(12, 4000)
@@ -30,7 +31,8 @@ def test_seeded_and_deterministic(nb, name):
         assert np.all(np.isfinite(x))
 
 
-@pytest.mark.parametrize("name", ["ic_cube", "ic_cube_surface", "ic_sphere_surface", "ic_galaxy"])
+@pytest.mark.parametrize("name", ["ic_cube", "ic_cube_surface", "ic_sphere_surface", "ic_galaxy",
+                                  "ic_ball_world"])
 def test_arguments_checked(nb, name):
     with pytest.raises(nb.NBodyError):
         getattr(nb, name)(1, 10, -1.0)
@@ -190,3 +192,34 @@ def test_galaxy_style_from_seed(nb):
         h = pos[:, 2] - W / 2
         seen.add(0 if h.min() >= 0.1 else 1)
     assert seen == {0, 1}
+
+
+def test_ball(nb):
+    # ParticleSystemBall.cpp:19-89: uniform in the ball of radius R = |w|/3 about
+    # the world centre, each coordinate nudged by U(-1, 1) * 0.05 R
+    n = 1_000_000
+    pos, vel, m = nb.ic_ball_world(8, n)
+    R = np.sqrt(3.0) * W / 3.0
+    d = 0.05 * R
+    rel = pos - W / 2
+    assert np.abs(rel).max() <= R + d
+    r2 = (rel * rel).sum(1)
+    assert np.sqrt(r2.max()) <= R + np.sqrt(3.0) * d
+    for k in range(3):
+        assert zscore(rel[:, k].mean(), 0.0, R * R / 5 + d * d / 3, n) < 5
+    # E|x|^2 = 3R^2/5 for the uniform ball, plus d^2 from the nudge (3 axes of
+    # variance d^2/3); a missing nudge is about 9 standard deviations away, a
+    # nudge of 5% of w instead of R about 20
+    var_r2 = R ** 4 * (3.0 / 7.0 - 9.0 / 25.0)
+    assert zscore(r2.mean(), 0.6 * R * R + d * d, var_r2, n) < 5
+    assert (r2 > R * R).any()  # the nudge pushes some bodies past R
+    s = 0.001 * np.sqrt(3.0) * W
+    assert np.abs(vel).max() <= s
+    for k in range(3):
+        assert zscore(vel[:, k].mean(), 0.0, s * s / 3, n) < 5
+        assert abs(vel[:, k].var() / (s * s / 3) - 1.0) < 0.01
+    # masses (w^3/n) (0.5 + 4.5 u^2): in [0.5, 5] x the mean mass, average 2x
+    avg = W ** 3 / n
+    m64 = m.astype(np.float64)
+    assert m64.min() >= 0.5 * avg * (1 - 1e-6) and m64.max() <= 5.0 * avg * (1 + 1e-6)
+    assert zscore(m64.mean(), 2.0 * avg, (4.5 * avg) ** 2 * 4.0 / 45.0, n) < 5
