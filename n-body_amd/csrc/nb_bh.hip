@@ -1026,10 +1026,11 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ pms
   }
 }
 
-// Wave traversal (the default). The 64 lanes of a wave hold 64 consecutive
-// Morton-sorted bodies and share every decision: a cell is accepted for the
-// whole wave when side^2 < theta^2 dmin^2, dmin = distance from the cell's COM
-// to the wave's bounding box; otherwise it is opened, and a leaf that is not
+// Wave traversal (the default). A wave takes one range of at most
+// NB_BH_WAVE_BODIES Morton-consecutive bodies of one cell (emitted by the build)
+// and its lanes share every decision: a cell is accepted for the whole range
+// when side^2 < theta^2 dmin^2, dmin = distance from the cell's COM to the
+// range's bounding box; otherwise it is opened, and a leaf that is not
 // accepted is evaluated body by body by every lane. Per body this accepts a
 // subset of what the per-body test accepts (at least as accurate), and nothing
 // diverges. For latency, each wave keeps a stack of opened nodes in LDS as

@@ -141,7 +141,8 @@ int64_t or_bh_wave_ranges(int64_t n, int64_t nodes, const int32_t* first_child,
  * bounding box, else the group descends; a leaf that is not accepted is
  * evaluated body by body for every member. Per body this accepts a subset of
  * the cells the per-body test accepts, so it is at least as accurate. group is
- * the range capacity (64 on the GPU; 1 gives the per-body traversal). */
+ * the range capacity (NB_BH_WAVE_BODIES = 32 on the GPU; 1 gives the per-body
+ * traversal). */
 int64_t or_bh_forces_group(int64_t n, const float* pos4s, const uint32_t* perm,
                            const float bmin[3], const float bmax[3], int64_t nodes,
                            const int32_t* first_child, const int32_t* next, const int32_t* start,
