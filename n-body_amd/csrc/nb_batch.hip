@@ -498,6 +498,7 @@ int nb_batch_remove(nb_batch* b, int32_t id) {
 
 int nb_batch_step(nb_batch* b, int32_t nsteps) {
   NB_TRY
+  NB_RANGE("nb_batch_step");
   if (!b) return nb_fail(NB_E_ARG, "nb_batch_step: null batch");
   if (nsteps < 0) return nb_fail(NB_E_ARG, "nb_batch_step: nsteps < 0");
   {
@@ -572,6 +573,7 @@ static int copy_slices(nb_batch* b, int32_t id, const double* src, double* dst) 
 
 int nb_batch_forces(nb_batch* b, int32_t id, double* force3) {
   NB_TRY
+  NB_RANGE("nb_batch_forces");
   NB_API_SHARED;
   int rc = check_slot(b, id);
   if (rc) return rc;
@@ -661,6 +663,7 @@ int nb_batch_get_timing(nb_batch* b, nb_timing* out) {
 
 int nb_batch_tune(nb_batch* b, int32_t reps, nb_tuning* out) {
   NB_TRY
+  NB_RANGE("nb_batch_tune");
   NB_API_SHARED;
   if (!b) return nb_fail(NB_E_ARG, "nb_batch_tune: null batch");
   if (reps < 1) reps = 3;

@@ -166,6 +166,7 @@ extern "C" {
 
 int nb_create(nb_ctx** out, const nb_config* cfg) {
   NB_TRY
+  NB_RANGE("nb_create");
   NB_API_SHARED;
   if (!out || !cfg) return nb_fail(NB_E_ARG, "nb_create: null argument");
   *out = nullptr;
@@ -276,6 +277,7 @@ static int upload_common(nb_ctx* c, const std::vector<double>& p, const std::vec
 int nb_upload(nb_ctx* c, const float* pos4, const float* vel4, const float* acc4,
               const float* mass, int64_t n) {
   NB_TRY
+  NB_RANGE("nb_upload");
   NB_API_SHARED;
   if (!c || !pos4 || !vel4 || !mass) return nb_fail(NB_E_ARG, "nb_upload: null argument");
   if (n != c->n) return nb_fail(NB_E_ARG, "nb_upload: n differs from the context's n");
@@ -293,6 +295,7 @@ int nb_upload(nb_ctx* c, const float* pos4, const float* vel4, const float* acc4
 int nb_upload_f64(nb_ctx* c, const double* pos3, const double* vel3, const double* acc3,
                   const float* mass, int64_t n) {
   NB_TRY
+  NB_RANGE("nb_upload_f64");
   NB_API_SHARED;
   if (!c || !pos3 || !vel3 || !mass) return nb_fail(NB_E_ARG, "nb_upload_f64: null argument");
   if (n != c->n) return nb_fail(NB_E_ARG, "nb_upload_f64: n differs from the context's n");
@@ -309,6 +312,7 @@ int nb_upload_f64(nb_ctx* c, const double* pos3, const double* vel3, const doubl
 
 int nb_step(nb_ctx* c, int32_t nsteps) {
   NB_TRY
+  NB_RANGE("nb_step");
   if (!c) return nb_fail(NB_E_ARG, "nb_step: null context");
   if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_step: no particle system uploaded");
   if (nsteps < 0) return nb_fail(NB_E_ARG, "nb_step: nsteps < 0");
@@ -381,6 +385,7 @@ static int force_pass(nb_ctx* c) {
 
 int nb_forces(nb_ctx* c, float* force4) {
   NB_TRY
+  NB_RANGE("nb_forces");
   NB_API_SHARED;
   if (!c || !force4) return nb_fail(NB_E_ARG, "nb_forces: null argument");
   int rc = force_pass(c);
@@ -401,6 +406,7 @@ int nb_forces(nb_ctx* c, float* force4) {
 
 int nb_forces_f64(nb_ctx* c, double* force3) {
   NB_TRY
+  NB_RANGE("nb_forces_f64");
   NB_API_SHARED;
   if (!c || !force3) return nb_fail(NB_E_ARG, "nb_forces_f64: null argument");
   int rc = force_pass(c);
@@ -433,6 +439,7 @@ static int download_soa(nb_ctx* c, std::vector<double>& p, std::vector<double>& 
 
 int nb_download(nb_ctx* c, float* pos4, float* vel4, float* acc4) {
   NB_TRY
+  NB_RANGE("nb_download");
   NB_API_SHARED;
   if (!c) return nb_fail(NB_E_ARG, "nb_download: null context");
   std::vector<double> p, v, a;
@@ -450,6 +457,7 @@ int nb_download(nb_ctx* c, float* pos4, float* vel4, float* acc4) {
 
 int nb_download_f64(nb_ctx* c, double* pos3, double* vel3, double* acc3) {
   NB_TRY
+  NB_RANGE("nb_download_f64");
   NB_API_SHARED;
   if (!c) return nb_fail(NB_E_ARG, "nb_download_f64: null context");
   std::vector<double> p, v, a;
@@ -467,6 +475,7 @@ int nb_download_f64(nb_ctx* c, double* pos3, double* vel3, double* acc3) {
 
 int nb_energy(nb_ctx* c, double* ekin, double* epot, double p[3]) {
   NB_TRY
+  NB_RANGE("nb_energy");
   NB_API_SHARED;
   if (!c) return nb_fail(NB_E_ARG, "nb_energy: null context");
   if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_energy: no particle system uploaded");
@@ -563,6 +572,7 @@ static int ensure_tmp(nb_ctx* c) {
 
 int nb_reorder(nb_ctx* c) {
   NB_TRY
+  NB_RANGE("nb_reorder");
   NB_API_SHARED;
   if (!c) return nb_fail(NB_E_ARG, "nb_reorder: null context");
   if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_reorder: nothing uploaded");

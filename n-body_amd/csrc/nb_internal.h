@@ -11,6 +11,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "nbody.h"
 
 // ---- error plumbing -------------------------------------------------------
@@ -46,6 +48,16 @@ std::shared_mutex& nb_capture_mutex();
   catch (...) {                                                                       \
     return nb_fail(NB_E_HIP, fn ": unexpected exception");                            \
   }
+
+// Host trace range over an entry point (roctx): it names the call on a
+// `rocprofv3 --marker-trace` timeline and costs a few ns when no tool is attached.
+struct nb_trace_range {
+  explicit nb_trace_range(const char* name) { roctxRangePushA(name); }
+  ~nb_trace_range() { roctxRangePop(); }
+  nb_trace_range(const nb_trace_range&) = delete;
+  nb_trace_range& operator=(const nb_trace_range&) = delete;
+};
+#define NB_RANGE(name) nb_trace_range nb_trace_range_(name)
 
 // Padding granule of the FP32 body array: the direct kernel's i-blocks hold
 // 64 lanes x up to 4 bodies, and its j-range is split over up to 4 waves in

@@ -362,6 +362,7 @@ int nb_server_submit(nb_server* s, const nb_job* job, int64_t* id) {
 
 int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
   NB_TRY
+  NB_RANGE("nb_server_tick");
   if (!s) return nb_fail(NB_E_ARG, "nb_server_tick: null server");
   int finished = 0;
   // 0. Barnes-Hut jobs whose worker has copied them out are done

@@ -85,6 +85,7 @@ bool nb_tuned_shape(int64_t n, int* bpl, int* jsplit, int* leaf) {
 
 extern "C" int nb_autotune(int64_t n, int32_t what, int32_t reps, nb_tuning* out) {
   NB_TRY
+  NB_RANGE("nb_autotune");
   if (n < 1 || n >= (int64_t(1) << 27)) return nb_fail(NB_E_ARG, "nb_autotune: n out of range");
   if (reps < 1) reps = 3;
   const Input in = plummer(n);
