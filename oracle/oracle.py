@@ -212,7 +212,8 @@ def bh_forces(pos4s, perm, bmin, bmax, tree, com4, eps2, theta=0.5, G=1.0, nthre
 
 def bh_pipeline(pos4, eps2, theta=0.5, leaf_size=8, G=1.0, nthreads=None, group=0, cubic=False):
     """The engine's Barnes-Hut force pass on the CPU. The GPU's defaults are
-    group=64 (wave traversal) and cubic=True; group=0 is the per-body traversal."""
+    group=NB_BH_WAVE_BODIES (32, the wave traversal) and cubic=True; group=0 is
+    the per-body traversal."""
     pos4 = np.ascontiguousarray(pos4, np.float32)
     n = pos4.shape[0]
     out = np.zeros((n, 4), np.float32)

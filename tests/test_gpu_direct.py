@@ -215,3 +215,27 @@ def test_autotune_installs_defaults(nb, ora):
                          targets=tgt)
     assert np.percentile(rel_err(f[tgt], ref), 99) < 1e-5
 
+
+
+def test_c2_size_symmetries(nb):
+    """Size-independent properties at the full N = 2^18: the x -> -x mirror is
+    bitwise (every operation of the pass is odd in dx), a permutation of the
+    input permutes the forces up to the summation order, and the momentum
+    balance sum_i m_i a_i = 0 holds to FP32 pair rounding."""
+    n = 1 << 18
+    pos, vel, m = nb.ic_plummer(203, n, 50.0, 1e4)
+
+    def forces(p, mm):
+        with nb.Simulation(n, EPS2, DT) as sim:
+            sim.upload(p, vel, mm)
+            return sim.forces_f64()
+
+    f = forces(pos, m)
+    flip = np.array([-1.0, 1.0, 1.0])
+    np.testing.assert_array_equal(forces(pos * flip, m), f * flip)
+    perm = np.random.default_rng(4).permutation(n)
+    e = rel_err(forces(pos[perm], m[perm]), f[perm])
+    print(f"N=2^18 permuted input: p99={np.percentile(e, 99):.2e} max={e.max():.2e}")
+    assert np.percentile(e, 99) < 1e-6 and e.max() < 1e-5
+    ma = m.astype(np.float64)[:, None] * f
+    assert np.all(np.abs(ma.sum(0)) < 1e-6 * np.abs(ma).sum(0))
