@@ -188,6 +188,21 @@ def test_c3_discs_properties(nb, ora):
     assert np.median(e) < 1e-2
 
 
+def test_c3_mass_scaling_is_bitwise(nb):
+    """At the full N = 2^22: the tree depends on positions only, and every mass
+    term (FP64 prefix sums, node masses, the m r^-3 factor, the sums) scales by
+    2 exactly, so doubling every mass doubles every force bit for bit."""
+    n = 1 << 22
+    pos, vel, m = nb.ic_discs(2025, n)
+
+    def forces(mm):
+        with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH, leaf_size=16) as sim:
+            sim.upload(pos, vel, mm)
+            return sim.forces_f64()
+
+    np.testing.assert_array_equal(forces(2.0 * m), 2.0 * forces(m))
+
+
 def test_axis_box_matches_reference_quantisation(nb, ora):
     # NB_F_BH_AXIS_BOX: each axis over its own extent, the inputs of morton.glsl
     n = 6000

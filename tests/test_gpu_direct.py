@@ -220,8 +220,9 @@ def test_autotune_installs_defaults(nb, ora):
 def test_c2_size_symmetries(nb):
     """Size-independent properties at the full N = 2^18: the x -> -x mirror is
     bitwise (every operation of the pass is odd in dx), a permutation of the
-    input permutes the forces up to the summation order, and the momentum
-    balance sum_i m_i a_i = 0 holds to FP32 pair rounding."""
+    input permutes the forces up to the summation order, the momentum balance
+    sum_i m_i a_i = 0 holds to FP32 pair rounding, and doubling every mass
+    doubles every force bit for bit."""
     n = 1 << 18
     pos, vel, m = nb.ic_plummer(203, n, 50.0, 1e4)
 
@@ -239,3 +240,4 @@ def test_c2_size_symmetries(nb):
     assert np.percentile(e, 99) < 1e-6 and e.max() < 1e-5
     ma = m.astype(np.float64)[:, None] * f
     assert np.all(np.abs(ma.sum(0)) < 1e-6 * np.abs(ma).sum(0))
+    np.testing.assert_array_equal(forces(pos, 2.0 * m), 2.0 * f)  # every mass term scales exactly
