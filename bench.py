@@ -18,6 +18,9 @@ Also reported on the same line:
                 outside the timed steps); its roofline object
                 prices the traversal kernel at 20 flop per interaction counted on
                 the first timed step, over the mean traversal launch time;
+                ms_per_step_after_reorder times the same steps once more after
+                one Morton reorder of the resident state (nb_reorder: layout
+                only, same work, bitwise the same forces);
   batch_c4      512 independent Plummer clusters (N_c log-uniform in
                 [256, 32768]) in one persistent launch per step (--no-batch),
                 with the same roofline object for the batched force kernel;
@@ -213,6 +216,17 @@ def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0, accuracy: boo
                    "energy_steps": warmup + steps}
             log(f"bh: force error vs FP64 p50 {fe['p50']:.2e} p99 {fe['p99']:.2e}; "
                 f"energy drift {acc['energy_drift']:.2e} over {warmup + steps} steps")
+        # the same system after one Morton reorder of the resident state: the
+        # step does the same work, with coalesced gathers and force writes
+        sim.set_timing(False)
+        sim.reorder()
+        sim.step(warmup)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        sim.sync()
+        reordered_ms = 1e3 * (time.perf_counter() - t0) / steps
+        log(f"bh: {reordered_ms:.2f} ms/step after one Morton reorder")
     ms = 1e3 * wall / steps
     cpu = None
     if cpu_budget > 0:
@@ -224,6 +238,7 @@ def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0, accuracy: boo
         "leaf_size": nb.get_tuning(n)["leaf_size"],
         "steps": steps,
         "ms_per_step": ms,
+        "ms_per_step_after_reorder": reordered_ms,
         "build_ms": t["build_ms"] / max(1, t["build_calls"]),
         "traverse_ms": t["force_ms"] / max(1, t["force_calls"]),
         "interactions_first_step": inter,
