@@ -285,8 +285,7 @@ int rebuild_tables(nb_batch* b) {
 
 void free_batch(nb_batch* b) {
   if (!b) return;
-  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
+  nb_drop_graph(b->stream, b->graph_exec, b->graph);
   for (auto& e : b->ev)
     if (e) (void)hipEventDestroy(e);
   void* ps[] = {b->pm,     b->pos,    b->vel,   b->acc,   b->force, b->mass,  b->c_off,
@@ -698,10 +697,7 @@ int nb_batch_tune(nb_batch* b, int32_t reps, nb_tuning* out) {
   b->bpl = best_b;
   b->jsplit = best_k;
   b->dirty = true;
-  if (b->graph_exec) (void)hipGraphExecDestroy(b->graph_exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  b->graph_exec = nullptr;
-  b->graph = nullptr;
+  nb_drop_graph(b->stream, b->graph_exec, b->graph);
   int rc = rebuild_tables(b);
   if (rc) return rc;
   if (out) {

@@ -143,6 +143,16 @@ __device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
   return v;
 }
 
+// Inverse of expand_bits on 10-bit values: gathers bits 0, 3, 6, ... of v.
+__device__ __forceinline__ uint32_t compact_bits(uint32_t v) {
+  v &= 0x49249249u;
+  v = (v ^ (v >> 2)) & 0xC30C30C3u;
+  v = (v ^ (v >> 4)) & 0x0F00F00Fu;
+  v = (v ^ (v >> 8)) & 0xFF0000FFu;
+  v = (v ^ (v >> 16)) & 0x000003FFu;
+  return v;
+}
+
 // morton.glsl:68-75 and :49-51 with the extent given: n = clamp((p - lo) / ext, 0, 1),
 // q = uint(clamp(n * 1024, 0, 1023)).
 __device__ __forceinline__ uint32_t quantize(float p, float lo, float ext) {
@@ -866,6 +876,11 @@ __global__ __launch_bounds__(1024) void k_top_build(const int* __restrict__ top_
     if (n <= kWaveBodies) {
       waves[0] = make_int2(0, n);
       s_ranges = 1;
+    } else if (n <= leaf) {
+      // the root is itself a leaf, so no node packs ranges below it: chunk it
+      int r = 0;
+      for (int o = 0; o < n; o += kWaveBodies) waves[r++] = make_int2(o, min(kWaveBodies, n - o));
+      s_ranges = r;
     }
   }
   // 2. links, level by level (a node's rope falls back to its parent's)
@@ -916,30 +931,68 @@ __global__ __launch_bounds__(1024) void k_top_build(const int* __restrict__ top_
   if (tid == 0) wctl[0] = s_ranges;
 }
 
-// ---- 7. centre of mass ------------------------------------------------------------------
+// ---- 7. centre of mass and opening radius -------------------------------------------
+// Squared opening radius of the default traversal: r = s_l / theta + delta, where
+// s_l is the cell side of the reference's test (Octree.cpp:311-320) and delta the
+// distance from the COM to the cell's geometric centre. This is the offset term of
+// Barnes' modified criterion (J. Comput. Phys. 87, 1990; the default of his
+// treecode 1.4): a cell whose mass sits near one face can hold bodies much closer
+// to a target than s_l / theta suggests. The cell is the 3l-bit prefix of its first
+// body's code, so its centre is bmin + (index + 1/2) * extent / 2^l per axis.
+// Every operation is rounded to nearest in a fixed order, so the CPU build
+// computes the same bits from the same COM. Massless cells get -1: always
+// accepted, never listed.
+__device__ __forceinline__ float open_radius2(float4 cm, uint32_t code, int l,
+                                              const float* __restrict__ geom, int cubic,
+                                              float theta) {
+  if (!(cm.w > 0.f)) return -1.f;
+  const float E = geom[6 + kMaxLevel + 1];
+  const uint32_t q = code >> (30 - 3 * l);
+  const float cmv[3] = {cm.x, cm.y, cm.z};
+  float d2 = 0.f;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const float ext = cubic ? E : fmaxf(__fsub_rn(geom[3 + a], geom[a]), 1e-6f);
+    const float h = ldexpf(ext, -l);
+    const uint32_t idx = compact_bits(q >> (2 - a));
+    const float ctr = __fadd_rn(geom[a], __fmul_rn(__fadd_rn((float)idx, 0.5f), h));
+    const float d = __fsub_rn(cmv[a], ctr);
+    d2 = a == 0 ? __fmul_rn(d, d) : __fadd_rn(d2, __fmul_rn(d, d));
+  }
+  const float rc = __fadd_rn(__fdiv_rn(ldexpf(E, -l), theta), __fsqrt_rn(d2));
+  return __fmul_rn(rc, rc);
+}
+
 __global__ __launch_bounds__(256) void k_tree_com(const int4* __restrict__ link, int n,
                                                   int64_t max_nodes,
                                                   const int* __restrict__ level_off,
                                                   const double4* __restrict__ prefix,
                                                   const float4* __restrict__ pms,
-                                                  float4* __restrict__ com) {
+                                                  const uint32_t* __restrict__ codes,
+                                                  const float* __restrict__ geom, int cubic,
+                                                  float theta, float4* __restrict__ com,
+                                                  float* __restrict__ r2) {
   const int total = (int)min((int64_t)level_off[kMaxLevel + 1], max_nodes);
   for (int k = blockIdx.x * 256 + threadIdx.x; k < total; k += gridDim.x * 256) {
     const int4 nd = link[k];
     const int cnt = link_count(nd.w);
     if (nd.z < 0 || cnt < 1 || nd.z + cnt > n) {
       com[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+      r2[k] = -1.f;
       continue;
     }
+    float4 cm;
     if (cnt == 1) {
-      com[k] = pms[nd.z];
-      continue;
+      cm = pms[nd.z];
+    } else {
+      const double4 a = prefix[nd.z], b = prefix[nd.z + cnt];
+      const double m = b.x - a.x;
+      const double inv = m > 0.0 ? 1.0 / m : 0.0;
+      cm = make_float4((float)((b.y - a.y) * inv), (float)((b.z - a.z) * inv),
+                       (float)((b.w - a.w) * inv), (float)m);
     }
-    const double4 a = prefix[nd.z], b = prefix[nd.z + cnt];
-    const double m = b.x - a.x;
-    const double inv = m > 0.0 ? 1.0 / m : 0.0;
-    com[k] = make_float4((float)((b.y - a.y) * inv), (float)((b.z - a.z) * inv),
-                         (float)((b.w - a.w) * inv), (float)m);
+    com[k] = cm;
+    r2[k] = open_radius2(cm, codes[nd.z], link_level(nd.w), geom, cubic, theta);
   }
 }
 
@@ -1120,7 +1173,6 @@ struct WaveLds {
   uint32_t* list;
   float4* stage;
   const int* lvl;      // first node of each level (kMaxLevel + 2 entries)
-  const float* side2;  // cell side^2 per level
 };
 
 // Walk the tree for the bodies of one wave range (lanes past the range hold a
@@ -1128,8 +1180,8 @@ struct WaveLds {
 template <bool kCount, bool kGuard>
 __device__ void traverse_range(const WaveLds& L, int lane, int P, float4 me, int n,
                                const float4* __restrict__ pms, const float4* __restrict__ com,
-                               const int4* __restrict__ link, float theta2, float eps2,
-                               int* __restrict__ flags, float& ax, float& ay, float& az,
+                               const int4* __restrict__ link, const float* __restrict__ r2,
+                               float eps2, int* __restrict__ flags, float& ax, float& ay, float& az,
                                unsigned long long& cnt) {
   const uint64_t lt = (uint64_t(1) << lane) - 1;
   const int log2S = 6 - (31 - __clz(P)), slice = lane / P;  // S = 64 / P
@@ -1166,6 +1218,7 @@ __device__ void traverse_range(const WaveLds& L, int lane, int P, float4 me, int
     int child = -1;
     int4 cl = make_int4(-1, -1, 0, 0);
     float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+    float rr = 0.f;
     if (q < k) {
       const int2 e = stack[sp + q];
       const uint32_t w = (uint32_t)e.y;
@@ -1177,6 +1230,7 @@ __device__ void traverse_range(const WaveLds& L, int lane, int P, float4 me, int
         if (cl.z < pend) {
           child = cand;
           c = com[cand];
+          rr = r2[cand];
         }
       }
     }
@@ -1187,7 +1241,7 @@ __device__ void traverse_range(const WaveLds& L, int lane, int P, float4 me, int
       const float ey = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.f);
       const float ez = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.f);
       const float dmin2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
-      accept = L.side2[link_level(cl.w)] < theta2 * dmin2;
+      accept = rr < dmin2;  // the group's modified test (see open_radius2)
       leaf = !accept && cl.x < 0;
       open = !accept && cl.x >= 0;
     }
@@ -1239,22 +1293,20 @@ template <bool kCount, bool kGuard>
 __global__ __launch_bounds__(64 * kListWaves, kCount ? 5 : kListBlocksPerCU) void k_traverse_list(
     const float4* __restrict__ pms, const uint32_t* __restrict__ perm, int n,
     const float4* __restrict__ com, const int4* __restrict__ link,
-    const int* __restrict__ level_off, int64_t max_nodes, const float* __restrict__ geom,
-    float theta2, float eps2, double G, double* __restrict__ force,
+    const int* __restrict__ level_off, int64_t max_nodes, const float* __restrict__ r2,
+    float eps2, double G, double* __restrict__ force,
     unsigned long long* __restrict__ inter, int* __restrict__ flags,
     const int2* __restrict__ waves, int* __restrict__ wctl) {
   __shared__ int2 s_stack[kListWaves][kStackCap];
   __shared__ uint32_t s_list[kListWaves][kListCap];
   __shared__ float4 s_stage[kListWaves][64];
   __shared__ int s_lvl[kMaxLevel + 2];
-  __shared__ float s_side2[kMaxLevel + 1];
   if (threadIdx.x < kMaxLevel + 2)
     s_lvl[threadIdx.x] = (int)min((int64_t)level_off[threadIdx.x], max_nodes);
-  if (threadIdx.x <= kMaxLevel) s_side2[threadIdx.x] = geom[6 + threadIdx.x];
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const WaveLds L{s_stack[wave], s_list[wave], s_stage[wave], s_lvl, s_side2};
+  const WaveLds L{s_stack[wave], s_list[wave], s_stage[wave], s_lvl};
   const int nranges = min(wctl[0], n);
   unsigned long long cnt = 0;
   for (;;) {
@@ -1271,7 +1323,7 @@ __global__ __launch_bounds__(64 * kListWaves, kCount ? 5 : kListBlocksPerCU) voi
     const float4 me = pms[t];
     float ax = 0.f, ay = 0.f, az = 0.f;
     unsigned long long c = 0;
-    traverse_range<kCount, kGuard>(L, lane, P, me, n, pms, com, link, theta2, eps2, flags, ax,
+    traverse_range<kCount, kGuard>(L, lane, P, me, n, pms, com, link, r2, eps2, flags, ax,
                                    ay, az, c);
     // add the slices' partial sums (butterfly over the slice bits, fixed order)
     for (int off = P; off < 64; off <<= 1) {
@@ -1379,6 +1431,7 @@ int nb_bh_alloc(nb_ctx* c) {
       A((void**)&b.prefix, sizeof(double4) * (n + 1)) ||
       A((void**)&b.scan_part, sizeof(double4) * ((n + kScanTile - 1) / kScanTile + 2)) ||
       A((void**)&b.node_com, sizeof(float4) * bound) ||
+      A((void**)&b.node_r2, sizeof(float) * bound) ||
       A((void**)&b.node_link, sizeof(int4) * bound) ||
       A((void**)&b.level_off, sizeof(int) * (kMaxLevel + 2)) ||
       A((void**)&b.child_cnt, sizeof(int) * n) || A((void**)&b.child_off, sizeof(int) * n) ||
@@ -1400,7 +1453,7 @@ void nb_bh_free(nb_ctx* c) {
   void* ps[] = {b.bbox_part, b.geom,     b.keys[0],  b.keys[1],   b.vals[0],    b.vals[1],
                 b.codes,     b.digit_hist, b.pms,    b.prefix,    b.scan_part,  b.node_com,
                 b.node_link, b.level_off, b.child_cnt, b.child_off, b.iscan_part, b.counters, b.waves,
-                b.inter,     b.top_start, b.child_bounds};
+                b.inter,     b.top_start, b.child_bounds, b.node_r2};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   b = nb_bh_state{};
@@ -1489,9 +1542,10 @@ hipError_t nb_bh_build(nb_ctx* c) {
   }
   // 7. centre of mass
   const int cgrid = (int)std::max<int64_t>(1, std::min<int64_t>((b.max_nodes + 255) / 256, 8192));
+  const int cubic = (c->cfg.flags & NB_F_BH_AXIS_BOX) ? 0 : 1;
   hipLaunchKernelGGL(k_tree_com, dim3(cgrid), dim3(256), 0, s, b.node_link, n, b.max_nodes,
-                     b.level_off, b.prefix,
-                     b.pms, b.node_com);
+                     b.level_off, b.prefix, b.pms, codes_sorted, b.geom, cubic, c->cfg.theta,
+                     b.node_com, b.node_r2);
   return hipGetLastError();
 }
 
@@ -1507,7 +1561,9 @@ hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions) {
       c->cfg.eps2, c->G, c->force, b.inter, c->flags
   const bool guard = c->guard;
   const int gl = b.persist_blocks;  // persistent blocks pulling wave ranges
-#define NB_LIST_ARGS NB_TRAVERSE_ARGS, b.waves, b.counters
+#define NB_LIST_ARGS                                                                     \
+  b.pms, b.vals[0], n, b.node_com, b.node_link, b.level_off, b.max_nodes, b.node_r2,     \
+      c->cfg.eps2, c->G, c->force, b.inter, c->flags, b.waves, b.counters
   if (group && count_interactions && guard)
     hipLaunchKernelGGL((k_traverse_list<true, true>), dim3(gl), dim3(64 * kListWaves), 0, c->stream,
                        NB_LIST_ARGS);
@@ -1558,8 +1614,10 @@ extern "C" int nb_debug_tree(nb_ctx* c, nb_node_view* nodes, int64_t cap, int64_
   if (nodes && m > 0) {
     std::vector<int4> link(m);
     std::vector<float4> com(m);
+    std::vector<float> r2(m);
     NB_HIP(hipMemcpy(link.data(), c->bh.node_link, sizeof(int4) * m, hipMemcpyDeviceToHost));
     NB_HIP(hipMemcpy(com.data(), c->bh.node_com, sizeof(float4) * m, hipMemcpyDeviceToHost));
+    NB_HIP(hipMemcpy(r2.data(), c->bh.node_r2, sizeof(float) * m, hipMemcpyDeviceToHost));
     for (int64_t k = 0; k < m; ++k) {
       nb_node_view& v = nodes[k];
       v.com[0] = com[k].x;
@@ -1571,7 +1629,7 @@ extern "C" int nb_debug_tree(nb_ctx* c, nb_node_view* nodes, int64_t cap, int64_
       v.body_start = link[k].z;
       v.body_count = link[k].w >> 4;
       v.level = link[k].w & 15;
-      v.pad_ = 0;
+      v.open_r2 = r2[k];
     }
   }
   return NB_OK;

@@ -43,8 +43,7 @@ hipError_t dalloc(T** p, size_t count) {
 
 void free_ctx(nb_ctx* c) {
   if (!c) return;
-  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-  if (c->graph) (void)hipGraphDestroy(c->graph);
+  nb_drop_graph(c->stream, c->graph_exec, c->graph);
   for (auto& e : c->ev)
     if (e) (void)hipEventDestroy(e);
   nb_bh_free(c);
@@ -179,6 +178,11 @@ int nb_create(nb_ctx** out, const nb_config* cfg) {
     return nb_fail(NB_E_ARG, "nb_create: unknown solver");
   if (cfg->solver == NB_SOLVER_BH && cfg->n >= (int64_t(1) << 27))
     return nb_fail(NB_E_ARG, "nb_create: Barnes-Hut supports n < 2^27");
+  // The direct-sum and integration kernels address the [3][n] FP64 state with
+  // 32-bit indices (2 n + i), so 3 n must stay below 2^31.
+  constexpr int64_t kMaxDirectBodies = ((int64_t(1) << 31) - 1) / 3;  // 715827882
+  if (cfg->solver == NB_SOLVER_DIRECT && cfg->n > kMaxDirectBodies)
+    return nb_fail(NB_E_ARG, "nb_create: the direct sum supports n <= 715827882");
   if (cfg->integrator != NB_INTEG_KDK_LAGGED && cfg->integrator != NB_INTEG_EULER)
     return nb_fail(NB_E_ARG, "nb_create: unknown integrator");
   if (cfg->leaf_size < 0 || cfg->leaf_size > 64)
@@ -259,8 +263,7 @@ static int upload_common(nb_ctx* c, const std::vector<double>& p, const std::vec
                          const std::vector<double>& a, const float* mass) {
   const size_t n = c->n;
   // Drop any captured step: it would be replayed against stale assumptions otherwise.
-  if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; }
-  if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+  nb_drop_graph(c->stream, c->graph_exec, c->graph);
   NB_HIP(hipMemcpyAsync(c->pos, p.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   NB_HIP(hipMemcpyAsync(c->vel, v.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
   NB_HIP(hipMemcpyAsync(c->acc, a.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));

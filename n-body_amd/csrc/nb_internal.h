@@ -86,6 +86,7 @@ struct nb_bh_state {
   double4* scan_part = nullptr; // scan block partials
   // octree, breadth-first
   float4* node_com = nullptr;   // (cx, cy, cz, m)
+  float* node_r2 = nullptr;     // squared opening radius (s / theta + |COM - cell centre|)^2
   int4* node_link = nullptr;    // (first_child, next, start, count << 4 | level)
   int* level_off = nullptr;     // [kMaxLevel + 2] node index where each level begins
   int* child_cnt = nullptr;     // per node of the current level: non-empty children
@@ -141,6 +142,17 @@ struct nb_ctx {
   hipEvent_t ev[8] = {};
   nb_timing tim{};
 };
+
+// Release a captured step graph. Launches of it may still be queued on `s`,
+// and the executable graph owns the kernel-argument memory they read, so the
+// stream is drained first.
+inline void nb_drop_graph(hipStream_t s, hipGraphExec_t& exec, hipGraph_t& graph) {
+  if (exec || graph) (void)hipStreamSynchronize(s);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  exec = nullptr;
+  graph = nullptr;
+}
 
 // ---- launchers (return hipError_t of the launch) ---------------------------
 hipError_t nb_launch_direct(nb_ctx* c);  // pm -> force

@@ -69,7 +69,7 @@ class NodeView(C.Structure):
         ("body_start", C.c_int32),
         ("body_count", C.c_int32),
         ("level", C.c_int32),
-        ("pad_", C.c_int32),
+        ("open_r2", C.c_float),
     ]
 
 
@@ -82,7 +82,7 @@ NODE_DTYPE = np.dtype(
         ("body_start", np.int32),
         ("body_count", np.int32),
         ("level", np.int32),
-        ("pad_", np.int32),
+        ("open_r2", np.float32),
     ]
 )
 assert NODE_DTYPE.itemsize == C.sizeof(NodeView)

@@ -883,6 +883,13 @@ int64_t or_bh_wave_ranges(int64_t n, int64_t nodes, const int32_t* first_child,
     range_count[0] = (int32_t)n;
     return 1;
   }
+  if (nodes > 0 && first_child[0] < 0) { /* the root is a leaf: chunks of it */
+    for (int64_t o = 0; o < n; o += cap) {
+      range_start[r] = (int32_t)o;
+      range_count[r++] = (int32_t)(n - o < cap ? n - o : cap);
+    }
+    return r;
+  }
   for (int64_t k = 0; k < nodes; ++k) {
     if (first_child[k] < 0 || nbody[k] <= cap) continue;
     const int64_t end = (int64_t)start[k] + nbody[k];
@@ -908,20 +915,51 @@ int64_t or_bh_wave_ranges(int64_t n, int64_t nodes, const int32_t* first_child,
   return r;
 }
 
+uint32_t or_compact_bits(uint32_t v) {
+  /* inverse of or_expand_bits on 10-bit values: keep bits 0, 3, 6, ... */
+  v &= 0x49249249u;
+  v = (v ^ (v >> 2)) & 0xC30C30C3u;
+  v = (v ^ (v >> 4)) & 0x0F00F00Fu;
+  v = (v ^ (v >> 8)) & 0xFF0000FFu;
+  v = (v ^ (v >> 16)) & 0x000003FFu;
+  return v;
+}
+
+void or_tree_open_r2(int64_t nodes, const int32_t* start, const int32_t* level, const float* com4,
+                     const uint32_t* codes_sorted, const float bmin[3], const float bmax[3],
+                     int cubic, float theta, float* r2) {
+  const float E = root_side(bmin, bmax);
+  for (int64_t k = 0; k < nodes; ++k) {
+    const float* c = &com4[4 * k];
+    if (!(c[3] > 0.f)) {
+      r2[k] = -1.f; /* massless: never opened, never listed */
+      continue;
+    }
+    const int l = level[k];
+    const uint32_t q = codes_sorted[start[k]] >> (30 - 3 * l); /* the cell's 3l-bit prefix */
+    float d2 = 0.f;
+    for (int a = 0; a < 3; ++a) {
+      const float ext = cubic ? E : axis_extent(bmin[a], bmax[a]);
+      const float h = ldexpf(ext, -l);
+      const uint32_t idx = or_compact_bits(q >> (2 - a));
+      const float ctr = bmin[a] + ((float)idx + 0.5f) * h;
+      const float d = c[a] - ctr;
+      d2 = a == 0 ? d * d : d2 + d * d;
+    }
+    const float rc = ldexpf(E, -l) / theta + sqrtf(d2);
+    r2[k] = rc * rc;
+  }
+}
+
 int64_t or_bh_forces_group(int64_t n, const float* pos4s, const uint32_t* perm,
                            const float bmin[3], const float bmax[3], int64_t nodes,
                            const int32_t* first_child, const int32_t* next, const int32_t* start,
                            const int32_t* nbody, const int32_t* level, const float* com4,
-                           float eps2, float G, float theta, int group, float* force4,
-                           int nthreads) {
+                           const uint32_t* codes_sorted, int cubic, float eps2, float G,
+                           float theta, int group, float* force4, int nthreads) {
   if (group < 1 || group > 64) return -1;
-  float side2[11];
-  const float s0 = root_side(bmin, bmax);
-  for (int l = 0; l <= 10; ++l) {
-    const float s = ldexpf(s0, -l);
-    side2[l] = s * s;
-  }
-  const float theta2 = theta * theta;
+  float* r2 = (float*)malloc(sizeof(float) * (size_t)(nodes ? nodes : 1));
+  or_tree_open_r2(nodes, start, level, com4, codes_sorted, bmin, bmax, cubic, theta, r2);
   int32_t* rs = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
   int32_t* rc = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n ? n : 1));
   const int64_t ngroups = or_bh_wave_ranges(n, nodes, first_child, start, nbody, level, group, rs, rc);
@@ -945,7 +983,7 @@ int64_t or_bh_forces_group(int64_t n, const float* pos4s, const uint32_t* perm,
       float e[3];
       for (int a = 0; a < 3; ++a) e[a] = fmaxf(fmaxf(lo[a] - c[a], c[a] - hi[a]), 0.f);
       const float dmin2 = e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
-      if (side2[level[i]] < theta2 * dmin2) {
+      if (r2[i] < dmin2) {
         if (c[3] > 0.f) {
           for (int64_t t = t0; t < t1; ++t) {
             const float dx = c[0] - pos4s[4 * t], dy = c[1] - pos4s[4 * t + 1], dz = c[2] - pos4s[4 * t + 2];
@@ -993,6 +1031,7 @@ int64_t or_bh_forces_group(int64_t n, const float* pos4s, const uint32_t* perm,
   }
   free(rs);
   free(rc);
+  free(r2);
   return total;
 }
 
@@ -1019,8 +1058,8 @@ int64_t or_bh_pipeline_group(int64_t n, const float* pos4, float eps2, float G, 
   const int64_t nodes = or_build_tree(n, sorted, leaf_size, bound, fc, nx, st, nb, lv);
   or_tree_com(nodes, st, nb, n, pos4s, com);
   const int64_t inter =
-      group > 0 ? or_bh_forces_group(n, pos4s, perm, bmin, bmax, nodes, fc, nx, st, nb, lv, com, eps2,
-                                     G, theta, group, force4, nthreads)
+      group > 0 ? or_bh_forces_group(n, pos4s, perm, bmin, bmax, nodes, fc, nx, st, nb, lv, com,
+                                     sorted, cubic, eps2, G, theta, group, force4, nthreads)
                 : or_bh_forces(n, pos4s, perm, bmin, bmax, nodes, fc, nx, st, nb, lv, com, eps2, G,
                                theta, force4, nthreads);
   if (codes_out) memcpy(codes_out, codes, sizeof(uint32_t) * (size_t)n);

@@ -135,20 +135,37 @@ int64_t or_bh_forces(int64_t n, const float* pos4s, const uint32_t* perm, const 
 int64_t or_bh_wave_ranges(int64_t n, int64_t nodes, const int32_t* first_child,
                           const int32_t* start, const int32_t* nbody, const int32_t* level,
                           int cap, int32_t* range_start, int32_t* range_count);
+/* Inverse of or_expand_bits on 10-bit values (bits 0, 3, 6, ... gathered). */
+uint32_t or_compact_bits(uint32_t v);
+/* Squared opening radius of every node for the engine's default traversal:
+ * r = s_l / theta + delta, with s_l = root side / 2^l (the cell side of
+ * Octree.cpp:311-320) and delta = |COM - geometric centre of the cell| (the
+ * offset term of Barnes' modified criterion, J. Barnes, "A modified tree code:
+ * don't laugh; it runs", J. Comput. Phys. 87 (1990); also the default of his
+ * treecode 1.4). The cell of a node is the 3l-bit prefix of its first body's
+ * code; its centre is bmin + (index + 1/2) * extent / 2^l per axis (extent: the
+ * cubic box's largest side, or each axis's own as in or_morton). Massless nodes
+ * get -1 (never opened, never listed). Every operation is one IEEE FP32 op in
+ * this order, which the GPU restates with explicit round-to-nearest intrinsics. */
+void or_tree_open_r2(int64_t nodes, const int32_t* start, const int32_t* level, const float* com4,
+                     const uint32_t* codes_sorted, const float bmin[3], const float bmax[3],
+                     int cubic, float theta, float* r2);
 /* Group traversal (the engine's default on the GPU): the bodies of one wave
  * range share every decision: a cell is accepted for the group when
- * s_l^2 < theta^2 dmin^2, dmin = distance from the cell's COM to the range's
- * bounding box, else the group descends; a leaf that is not accepted is
- * evaluated body by body for every member. Per body this accepts a subset of
- * the cells the per-body test accepts, so it is at least as accurate. group is
- * the range capacity (NB_BH_WAVE_BODIES = 32 on the GPU; 1 gives the per-body
- * traversal). */
+ * r^2 < dmin^2, r the node's opening radius (or_tree_open_r2) and dmin the
+ * distance from the cell's COM to the range's bounding box, else the group
+ * descends; a leaf that is not accepted is evaluated body by body for every
+ * member. Since r > s_l / theta and dmin <= d for every member, each body
+ * accepts a subset of the cells the reference's per-body test s_l^2 < theta^2 d^2
+ * (or_bh_forces) accepts on the same tree. group is the range capacity
+ * (NB_BH_WAVE_BODIES = 32 on the GPU; 1 gives a per-body traversal with the
+ * modified test). codes_sorted and cubic locate each node's cell. */
 int64_t or_bh_forces_group(int64_t n, const float* pos4s, const uint32_t* perm,
                            const float bmin[3], const float bmax[3], int64_t nodes,
                            const int32_t* first_child, const int32_t* next, const int32_t* start,
                            const int32_t* nbody, const int32_t* level, const float* com4,
-                           float eps2, float G, float theta, int group, float* force4,
-                           int nthreads);
+                           const uint32_t* codes_sorted, int cubic, float eps2, float G,
+                           float theta, int group, float* force4, int nthreads);
 /* The whole engine BH force pass on the CPU: bbox, Morton, sort, tree, COM,
  * traversal. Node arrays are optional outputs (NULL to skip). */
 int64_t or_bh_pipeline(int64_t n, const float* pos4, float eps2, float G, float theta,

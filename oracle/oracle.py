@@ -37,8 +37,10 @@ _SIGS = {
     "or_tree_com": (None, [_I, _P, _P, _I, _P, _P]),
     "or_bh_forces": (_I, [_I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, C.c_float, C.c_float,
                           C.c_float, _P, C.c_int]),
-    "or_bh_forces_group": (_I, [_I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, C.c_float, C.c_float,
-                                C.c_float, C.c_int, _P, C.c_int]),
+    "or_bh_forces_group": (_I, [_I, _P, _P, _P, _P, _I, _P, _P, _P, _P, _P, _P, _P, C.c_int,
+                                C.c_float, C.c_float, C.c_float, C.c_int, _P, C.c_int]),
+    "or_compact_bits": (C.c_uint32, [C.c_uint32]),
+    "or_tree_open_r2": (None, [_I, _P, _P, _P, _P, _P, _P, C.c_int, C.c_float, _P]),
     "or_bh_wave_ranges": (_I, [_I, _I, _P, _P, _P, _P, C.c_int, _P, _P]),
     "or_bh_pipeline_group": (_I, [_I, _P, C.c_float, C.c_float, C.c_float, C.c_int, C.c_int, C.c_int,
                                   _P, _P, _P, _P, C.c_int]),
@@ -235,14 +237,33 @@ def bh_wave_ranges(tree, n, cap=64):
     return rs[:r].copy(), rc[:r].copy()
 
 
+def compact_bits(v: int) -> int:
+    return int(load().or_compact_bits(v))
+
+
+def tree_open_r2(tree, com4, codes_sorted, bmin, bmax, theta=0.5, cubic=True):
+    """Squared opening radius (s_l / theta + |COM - cell centre|)^2 per node."""
+    k = tree["start"].shape[0]
+    r2 = np.empty(k, np.float32)
+    load().or_tree_open_r2(k, _p(tree["start"]), _p(tree["level"]),
+                           _p(np.ascontiguousarray(com4, np.float32)),
+                           _p(np.ascontiguousarray(codes_sorted, np.uint32)),
+                           _p(np.asarray(bmin, np.float32)), _p(np.asarray(bmax, np.float32)),
+                           1 if cubic else 0, theta, _p(r2))
+    return r2
+
+
 def bh_forces_group(pos4s, perm, bmin, bmax, tree, com4, eps2, theta=0.5, G=1.0, group=64,
-                    nthreads=None):
+                    nthreads=None, *, codes_sorted, cubic=True):
+    """The engine's default traversal on the CPU (groups of `group` sorted
+    bodies, the modified opening test of or_tree_open_r2)."""
     n = pos4s.shape[0]
     out = np.zeros((n, 4), np.float32)
     inter = load().or_bh_forces_group(
         n, _p(np.ascontiguousarray(pos4s, np.float32)), _p(np.ascontiguousarray(perm, np.uint32)),
         _p(np.asarray(bmin, np.float32)), _p(np.asarray(bmax, np.float32)), tree["start"].shape[0],
         _p(tree["first_child"]), _p(tree["next"]), _p(tree["start"]), _p(tree["count"]),
-        _p(tree["level"]), _p(np.ascontiguousarray(com4, np.float32)), eps2, G, theta, group, _p(out),
-        nthreads or threads())
+        _p(tree["level"]), _p(np.ascontiguousarray(com4, np.float32)),
+        _p(np.ascontiguousarray(codes_sorted, np.uint32)), 1 if cubic else 0, eps2, G, theta, group,
+        _p(out), nthreads or threads())
     return out, int(inter)

@@ -212,9 +212,12 @@ void oracle() {
           or_bh_wave_ranges(n, cnt, fc.data(), st.data(), nb.data(), lv.data(), NB_BH_WAVE_BODIES,
                             rs.data(), rc.data());
       expect(ranges > 0 && ranges <= n, "wave ranges");
+      std::vector<float> r2(cnt);
+      or_tree_open_r2(cnt, st.data(), lv.data(), com.data(), sorted.data(), bmin, bmax, 1, 0.5f,
+                      r2.data());
       or_bh_forces_group(n, p4s.data(), perm.data(), bmin, bmax, cnt, fc.data(), nx.data(),
-                         st.data(), nb.data(), lv.data(), com.data(), 0.01f, 1.0f, 0.5f,
-                         NB_BH_WAVE_BODIES, f4.data(), 2);
+                         st.data(), nb.data(), lv.data(), com.data(), sorted.data(), 1, 0.01f, 1.0f,
+                         0.5f, NB_BH_WAVE_BODIES, f4.data(), 2);
     }
   }
   // the pipelines take (x, y, z, m) bodies
@@ -228,6 +231,12 @@ void oracle() {
   std::vector<float> one(4, 0.f), one_f(4);
   one[3] = 1.f;
   or_bh_pipeline(1, one.data(), 0.01f, 1.0f, 0.5f, 1, one_f.data(), nullptr, nullptr, nullptr, 1);
+  // fewer bodies than the largest leaf: the root is a leaf, cut into several ranges
+  std::vector<float> fifty(4 * 50), fifty_f(4 * 50);
+  for (int64_t i = 0; i < 50; ++i)
+    for (int c = 0; c < 4; ++c) fifty[4 * i + c] = c == 3 ? 1.f : pm[4 * i + c];
+  or_bh_pipeline_group(50, fifty.data(), 0.01f, 1.0f, 0.5f, 64, NB_BH_WAVE_BODIES, 1, fifty_f.data(),
+                       nullptr, nullptr, nullptr, 1);
   std::vector<float> same(4 * 64, 0.f);
   for (int64_t i = 0; i < 64; ++i) same[4 * i + 3] = 1.f;
   std::vector<float> same_f(4 * 64);

@@ -54,12 +54,17 @@ def test_argument_errors(nb):
            dict(n=8, bodies_per_lane=3), dict(n=8, j_split=32), dict(n=8, j_split=3),
            dict(n=8, bodies_per_lane=4, j_split=8), dict(n=8, bodies_per_lane=2, j_split=16),
            dict(n=8, solver=7),
-           dict(n=1 << 27, solver=1)]
+           dict(n=1 << 27, solver=1),
+           # direct sum: the kernels index the [3][n] state in int32, so 3 n must fit
+           dict(n=715827883, solver=0)]
     for kw in bad:
         cfg = nb.Config(**{"eps2": 1.0, "dt": 0.01, **kw})
         assert lib.nb_create(ctypes.byref(ctx), ctypes.byref(cfg)) == nb.NB_E_ARG, kw
         assert lib.nb_last_error()
     assert lib.nb_step(None, 1) == nb.NB_E_ARG
+    cfg = nb.Config(n=715827883, eps2=1.0, dt=0.01)
+    lib.nb_create(ctypes.byref(ctx), ctypes.byref(cfg))
+    assert b"715827882" in lib.nb_last_error()
 
 
 def test_initial_conditions_are_seeded(nb):

@@ -25,6 +25,13 @@ def cpu_build(ora, pos4, leaf, cubic=True):
     return mn, mx, codes, perm, tree, com
 
 
+def gpu_com(nodes):
+    com = np.zeros((len(nodes), 4), np.float32)
+    com[:, :3] = nodes["com"]
+    com[:, 3] = nodes["mass"]
+    return com
+
+
 def gpu_run(nb, pos, vel, m, leaf, theta=0.5, flags=0):
     with nb.Simulation(len(m), EPS2, DT, solver=nb.SOLVER_BH, leaf_size=leaf, theta=theta,
                        flags=flags) as sim:
@@ -63,9 +70,16 @@ def test_bit_exact_topology_and_forces(nb, ora, leaf):
     np.testing.assert_array_equal(nodes["level"], tree["level"])
     np.testing.assert_allclose(nodes["mass"], com[:, 3], rtol=1e-6)
     np.testing.assert_allclose(nodes["com"], com[:, :3], rtol=1e-6, atol=1e-5)
+    # opening radii: bit for bit the CPU formula applied to the GPU's own COMs,
+    # and within rounding of the CPU build's
+    srt = ccodes[cperm]
+    r2 = ora.tree_open_r2(tree, gpu_com(nodes), srt, mn, mx, theta=0.5)
+    np.testing.assert_array_equal(nodes["open_r2"], r2)
+    np.testing.assert_allclose(nodes["open_r2"], ora.tree_open_r2(tree, com, srt, mn, mx),
+                               rtol=1e-5)
     # default wave traversal vs its CPU twin (groups of NB_BH_WAVE_BODIES sorted bodies)
     fc, cinter = ora.bh_forces_group(p4[cperm], cperm, mn, mx, tree, com, EPS2,
-                                     group=nb.BH_WAVE_BODIES)
+                                     group=nb.BH_WAVE_BODIES, codes_sorted=srt)
     e = rel_err(f, fc)
     print(f"leaf {leaf}: GPU vs CPU build p50={np.median(e):.2e} p99={np.percentile(e, 99):.2e}; "
           f"interactions {inter} vs {cinter}")
@@ -117,6 +131,18 @@ def test_small_and_ragged(nb, ora, n):
     assert np.all(np.isfinite(f))
     if n == 1:
         assert np.all(f == 0)
+
+
+@pytest.mark.parametrize("n,leaf", [(50, 64), (64, 64), (33, 40)])
+def test_root_leaf_with_several_ranges(nb, ora, n, leaf):
+    # more bodies than one range holds but no more than one leaf: the root is a
+    # leaf cut into chunks, and every pair is evaluated
+    pos, vel, m = nb.ic_ball(70 + n, n, 10.0, 1.0)
+    f, codes, perm, nodes, bbox, inter = gpu_run(nb, pos, vel, m, leaf)
+    assert len(nodes) == 1 and nodes["first_child"][0] == -1
+    assert inter == n * (n - 1)
+    ref = ora.direct_f64(pos.astype(np.float32).astype(np.float64), m.astype(np.float64), EPS2)
+    assert rel_err(f, ref).max() < 1e-5
 
 
 def test_all_bodies_coincident(nb):
@@ -213,8 +239,11 @@ def test_axis_box_matches_reference_quantisation(nb, ora):
     np.testing.assert_array_equal(codes, ccodes)
     np.testing.assert_array_equal(perm, cperm)
     np.testing.assert_array_equal(nodes["next"], tree["next"])
+    srt = ccodes[cperm]
+    r2 = ora.tree_open_r2(tree, gpu_com(nodes), srt, mn, mx, theta=0.5, cubic=False)
+    np.testing.assert_array_equal(nodes["open_r2"], r2)
     fc, cinter = ora.bh_forces_group(p4[cperm], cperm, mn, mx, tree, com, EPS2,
-                                     group=nb.BH_WAVE_BODIES)
+                                     group=nb.BH_WAVE_BODIES, codes_sorted=srt, cubic=False)
     assert np.percentile(rel_err(f, fc), 99) < 1e-5
     # the cubic default opens fewer cells on flat systems
     _, _, _, _, _, inter_cubic = gpu_run(nb, pos, vel, m, 8)

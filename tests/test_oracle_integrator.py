@@ -43,6 +43,46 @@ def test_euler_hand_values(ora):
     np.testing.assert_array_equal(pos[0, :3], [0.75, 1.0, 1.0])
 
 
+def test_euler_f64_hand_values(ora):
+    # updateParticles.glsl:44-55: v += F dt first, then x += v dt with the new v;
+    # the stored acceleration becomes F
+    pos = np.array([[1.0, 0.0, 0.0]])
+    vel = np.array([[0.0, 2.0, 0.0]])
+    acc = np.array([[9.0, 9.0, 9.0]])
+    f = np.array([[-1.0, 0.0, 4.0]])
+    ora.euler_update_f64(pos, vel, acc, f, 0.5)
+    np.testing.assert_array_equal(vel[0], [-0.5, 2.0, 2.0])
+    np.testing.assert_array_equal(pos[0], [0.75, 1.0, 1.0])
+    np.testing.assert_array_equal(acc[0], [-1.0, 0.0, 4.0])
+
+
+def test_euler_f64_constant_force_closed_form(ora):
+    # Under a constant force the semi-implicit update gives v_k = v0 + k F dt and
+    # x_k = x0 + k v0 dt + F dt^2 k (k + 1) / 2 (explicit Euler would give k (k - 1) / 2)
+    x0, v0, F, dt, k = np.array([[3.0, -1.0, 0.5]]), np.array([[0.25, 0.0, -2.0]]), \
+        np.array([[1.0, -3.0, 0.125]]), 0.125, 12
+    pos, vel, acc = x0.copy(), v0.copy(), np.zeros((1, 3))
+    for _ in range(k):
+        ora.euler_update_f64(pos, vel, acc, F, dt)
+    np.testing.assert_array_equal(vel, v0 + k * F * dt)
+    np.testing.assert_array_equal(pos, x0 + k * v0 * dt + F * dt * dt * k * (k + 1) / 2)
+
+
+def test_euler_f64_matches_f32_on_exact_inputs(ora):
+    # inputs with few significant bits and dt = 1/8: every product and sum is
+    # exact in FP32 and FP64 alike, so the two restatements agree bit for bit
+    rng = np.random.default_rng(3)
+    n = 200
+    p, v, f = (rng.integers(-1000, 1000, (n, 3)) / 16.0 for _ in range(3))
+    p4, v4, f4 = vec4(p), vec4(v), vec4(f)
+    ora.euler_update(p4, v4, f4, 0.125)
+    p64, v64, a64 = p.copy(), v.copy(), np.zeros((n, 3))
+    ora.euler_update_f64(p64, v64, a64, f, 0.125)
+    np.testing.assert_array_equal(p4[:, :3].astype(np.float64), p64)
+    np.testing.assert_array_equal(v4[:, :3].astype(np.float64), v64)
+    np.testing.assert_array_equal(a64, f)
+
+
 def test_kdk_f64_matches_f32(ora):
     rng = np.random.default_rng(1)
     n = 100

@@ -263,14 +263,96 @@ def test_group_traversal_limits_and_accuracy(ora):
     # theta -> 0: every leaf, every pair
     f0, *_, inter0 = ora.bh_pipeline(pos, 40.0, theta=1e-7, leaf_size=8, group=64)
     assert inter0 == 3000 * 2999 and rel_err(f0, ref).max() < 1e-5
-    # the group test is stricter than the per-body test: at least as accurate, more pairs
-    fb, *_, ib = ora.bh_pipeline(pos, 40.0, theta=0.5, leaf_size=8, group=0)
-    fg, *_, ig = ora.bh_pipeline(pos, 40.0, theta=0.5, leaf_size=8, group=64)
-    assert ig >= ib
-    assert np.median(rel_err(fg, ref)) <= np.median(rel_err(fb, ref))
-    # group of one body is the per-body traversal
-    f1, *_, i1 = ora.bh_pipeline(pos, 40.0, theta=0.5, leaf_size=8, group=1)
-    assert i1 == ib and rel_err(f1, fb).max() < 1e-6
+    # the modified test accepts, per body, a subset of what the reference's test
+    # accepts on the same tree: more pairs, and at least as accurate
+    for cubic in (False, True):
+        fb, *_, ib = ora.bh_pipeline(pos, 40.0, theta=0.5, leaf_size=8, group=0, cubic=cubic)
+        f1, *_, i1 = ora.bh_pipeline(pos, 40.0, theta=0.5, leaf_size=8, group=1, cubic=cubic)
+        fg, *_, ig = ora.bh_pipeline(pos, 40.0, theta=0.5, leaf_size=8, group=64, cubic=cubic)
+        assert ig >= i1 > ib
+        e_b, e_1, e_g = (np.median(rel_err(f, ref)) for f in (fb, f1, fg))
+        assert e_g <= e_1 < e_b
+
+
+def test_modified_test_is_a_subset_of_the_reference_test(ora):
+    """Per body and per cell: whenever the modified test accepts a cell
+    ((s / theta + delta)^2 < d^2), the reference's test (s^2 < theta^2 d^2,
+    Octree.cpp:320) accepts it too, since delta >= 0."""
+    n = 2000
+    pos = vec4(seeded_ball(n, seed=41))
+    pos[:, 3] = 1.0
+    mn, mx = ora.bbox(pos)
+    perm, srt = ora.sort_codes(ora.morton(pos, mn, mx, cubic=True))
+    t = ora.build_tree(srt, 4)
+    com = ora.tree_com(t, pos[perm])
+    r2 = ora.tree_open_r2(t, com, srt, mn, mx, theta=0.5)
+    side = float(np.max(np.maximum(mx - mn, 1e-6)))
+    s2 = (side / 2.0 ** t["level"]) ** 2
+    d2 = ((com[None, :, :3].astype(np.float64) - pos[:200, None, :3]) ** 2).sum(-1)
+    accept_mod = r2[None, :] < d2
+    accept_ref = s2[None, :] < 0.25 * d2
+    assert accept_mod.any() and not (accept_mod & ~accept_ref).any()
+
+
+def test_compact_bits_inverts_expand_bits(ora):
+    for v in range(1024):
+        assert ora.compact_bits(ora.expand_bits(v)) == v
+    # and it reads every third bit only
+    assert ora.compact_bits(0xFFFFFFFF) == 0x3FF
+    assert ora.compact_bits(0x92492492) == 0 and ora.compact_bits(0x24924924) == 0
+
+
+@pytest.mark.parametrize("cubic", [True, False])
+def test_open_radius_from_the_cell_geometry(ora, cubic):
+    """r^2 = (s_l / theta + |COM - centre|)^2 with the centre found
+    independently: the cell of level l holding a node's first body is
+    floor(q / 2^(10-l)) with q the body's 10-bit quantised coordinate. Every
+    body of the node lies in that cell."""
+    rng = np.random.default_rng(42)
+    n = 3000
+    pos = vec4(rng.uniform(-1, 1, (n, 3)) * np.array([80.0, 30.0, 5.0]))
+    pos[:, 3] = rng.uniform(0.5, 2.0, n)
+    pos[::97, 3] = 0.0  # a few massless bodies
+    pos[5:25, :3] = pos[5, :3]  # and one massless-free coincident clump
+    mn, mx = ora.bbox(pos)
+    perm, srt = ora.sort_codes(ora.morton(pos, mn, mx, cubic=cubic))
+    t = ora.build_tree(srt, 3)
+    ps = pos[perm]
+    com = ora.tree_com(t, ps)
+    theta = 0.6
+    r2 = ora.tree_open_r2(t, com, srt, mn, mx, theta=theta, cubic=cubic)
+    ext = np.maximum(mx.astype(np.float64) - mn, 1e-6)
+    side = ext.max()
+    ext = np.full(3, side) if cubic else ext
+    q = np.clip(np.floor(np.clip((ps[:, :3] - mn) / ext, 0, 1) * 1024), 0, 1023).astype(np.int64)
+    for k in range(len(t["start"])):
+        lv, st, c = t["level"][k], t["start"][k], t["count"][k]
+        if com[k, 3] <= 0:
+            assert r2[k] == -1.0
+            continue
+        idx = q[st] >> (10 - lv)
+        assert np.all(q[st:st + c] >> (10 - lv) == idx)  # the node's bodies share the cell
+        h = ext / 2.0 ** lv
+        ctr = mn + (idx + 0.5) * h
+        want = (side / 2.0 ** lv / theta + np.linalg.norm(com[k, :3] - ctr)) ** 2
+        assert abs(r2[k] - want) <= 1e-5 * want
+
+
+def test_root_leaf_is_cut_into_ranges(ora):
+    """Fewer bodies than the leaf size but more than one range: the root is a
+    leaf and is cut into chunks; the group traversal then evaluates every pair."""
+    n = 50
+    pos = vec4(seeded_ball(n, seed=43))
+    pos[:, 3] = 1.0
+    mn, mx = ora.bbox(pos)
+    perm, srt = ora.sort_codes(ora.morton(pos, mn, mx, cubic=True))
+    t = ora.build_tree(srt, 64)
+    assert len(t["start"]) == 1 and t["first_child"][0] == -1
+    rs, rc = ora.bh_wave_ranges(t, n, 32)
+    assert list(rs) == [0, 32] and list(rc) == [32, 18]
+    ref = ora.direct_f64(pos[:, :3].astype(np.float64), np.ones(n), 40.0)
+    f, *_, inter = ora.bh_pipeline(pos, 40.0, theta=0.5, leaf_size=64, group=32, cubic=True)
+    assert inter == n * (n - 1) and rel_err(f, ref).max() < 1e-5
 
 
 def test_cubic_box_is_the_reference_formula_on_a_cube(ora):
@@ -305,10 +387,35 @@ def test_wave_ranges_partition_the_bodies(ora):
     for a, c in zip(rs, rc):
         cover[a:a + c] += 1
     assert np.all(cover == 1)
-    # each range lies in one cell: its bodies share a code prefix at the parent's level
+    # each range lies in one cell: find the deepest node holding the whole range;
+    # the range's codes share that node's prefix, and unless the node is a leaf
+    # (a chunk of it), the range is a run of whole children of a node that holds
+    # more than the capacity
+    fc, st, cnt, lv = t["first_child"], t["start"], t["count"], t["level"]
     for a, c in zip(rs, rc):
-        x = srt[a] ^ srt[a + c - 1]
-        assert x < (1 << 30)
+        k = 0
+        while fc[k] >= 0:
+            j = fc[k]
+            inside = None
+            while j < len(st) and lv[j] == lv[k] + 1 and st[j] < st[k] + cnt[k]:
+                if st[j] <= a and a + c <= st[j] + cnt[j]:
+                    inside = j
+                j += 1
+            if inside is None:
+                break
+            k = inside
+        shift = 30 - 3 * int(lv[k])
+        assert (int(srt[a]) >> shift) == (int(srt[a + c - 1]) >> shift)
+        if a == st[k] and c == cnt[k]:
+            continue  # the range is one whole cell
+        assert cnt[k] > 64  # a part of a larger node: a run of its children, or a chunk of a leaf
+        if fc[k] >= 0:
+            j, starts, ends = fc[k], set(), set()
+            while j < len(st) and lv[j] == lv[k] + 1 and st[j] < st[k] + cnt[k]:
+                starts.add(int(st[j]))
+                ends.add(int(st[j] + cnt[j]))
+                j += 1
+            assert int(a) in starts and int(a + c) in ends
     # a small system is one range; capacity 1 gives singletons
     rs1, rc1 = ora.bh_wave_ranges(ora.build_tree(srt[:50], 8), 50, 64)
     assert list(rs1) == [0] and list(rc1) == [50]
