@@ -939,12 +939,16 @@ __global__ __launch_bounds__(1024) void k_top_build(const int* __restrict__ top_
 // treecode 1.4): a cell whose mass sits near one face can hold bodies much closer
 // to a target than s_l / theta suggests. The cell is the 3l-bit prefix of its first
 // body's code, so its centre is bmin + (index + 1/2) * extent / 2^l per axis.
-// Every operation is rounded to nearest in a fixed order, so the CPU build
-// computes the same bits from the same COM. Massless cells get -1: always
-// accepted, never listed.
+// Every operation is one IEEE operation rounded to nearest, in a fixed order and
+// with contraction into FMAs off (HIP's __fadd_rn and __fmul_rn are plain
+// operators that -ffp-contract=fast may fuse, and its __fsqrt_rn is the native
+// approximation; division and sqrtf are correctly rounded by default), so the
+// CPU build computes the same bits from the same COM. Massless cells get -1:
+// always accepted, never listed.
 __device__ __forceinline__ float open_radius2(float4 cm, uint32_t code, int l,
                                               const float* __restrict__ geom, int cubic,
                                               float theta) {
+#pragma clang fp contract(off)
   if (!(cm.w > 0.f)) return -1.f;
   const float E = geom[6 + kMaxLevel + 1];
   const uint32_t q = code >> (30 - 3 * l);
@@ -952,15 +956,15 @@ __device__ __forceinline__ float open_radius2(float4 cm, uint32_t code, int l,
   float d2 = 0.f;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
-    const float ext = cubic ? E : fmaxf(__fsub_rn(geom[3 + a], geom[a]), 1e-6f);
+    const float ext = cubic ? E : fmaxf(geom[3 + a] - geom[a], 1e-6f);
     const float h = ldexpf(ext, -l);
     const uint32_t idx = compact_bits(q >> (2 - a));
-    const float ctr = __fadd_rn(geom[a], __fmul_rn(__fadd_rn((float)idx, 0.5f), h));
-    const float d = __fsub_rn(cmv[a], ctr);
-    d2 = a == 0 ? __fmul_rn(d, d) : __fadd_rn(d2, __fmul_rn(d, d));
+    const float ctr = geom[a] + ((float)idx + 0.5f) * h;
+    const float d = cmv[a] - ctr;
+    d2 = a == 0 ? d * d : d2 + d * d;
   }
-  const float rc = __fadd_rn(__fdiv_rn(ldexpf(E, -l), theta), __fsqrt_rn(d2));
-  return __fmul_rn(rc, rc);
+  const float rc = ldexpf(E, -l) / theta + sqrtf(d2);
+  return rc * rc;
 }
 
 __global__ __launch_bounds__(256) void k_tree_com(const int4* __restrict__ link, int n,
