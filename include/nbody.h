@@ -163,6 +163,33 @@ int nb_get_order(nb_ctx* ctx, int32_t* order);
  * which this engine leaves to the consumer. */
 int nb_render_view(nb_ctx* ctx, float* xyzs);
 
+/* ---- allocation-free Barnes-Hut force pass --------------------------------
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under hipGraph stream capture, tight serving loops, harnesses
+ * that forbid hipMalloc): the force pass of a Barnes-Hut context (the kernels of
+ * nb_forces / nb_forces_f64 with NB_SOLVER_BH, same results bit for bit) on
+ * caller-owned device buffers, with every intermediate array (Morton keys, sort
+ * buffers, octree, wave ranges, counters) in one static device array of the
+ * library, about 21 MB per GPU, held until the process ends. It calls no
+ * hipMalloc, hipFree, hipMemset or allocating library routine and does not
+ * synchronise: it only enqueues kernels on `stream` (a hipStream_t; NULL = the
+ * null stream) and returns.
+ *   pm4      device float[4 n]: (x, y, z, m) per body
+ *   force3   device double[3][n] (structure of arrays: fx[0..n), fy, fz):
+ *            G sum_j m_j r_ij / (|r_ij|^2 + eps2)^{3/2}, the FP32 sums widened
+ *   status   device int32 (may be NULL): zeroed, then OR-ed with 2 / 4 / 8 on
+ *            octree overflow / a walk leaving the node range / stack overflow
+ *   leaf_size 1..64 (0 = 8), theta (0 = 0.5), flags NB_F_BH_AXIS_BOX /
+ *            NB_F_BH_PER_BODY as for nb_create.
+ * 1 <= n <= NB_NOALLOC_MAX_BODIES, else NB_E_ARG with nothing enqueued. Every
+ * call on one GPU uses that GPU's one scratch array, which keeps the last
+ * call's contents: calls that overlap on one GPU (two streams, or two host
+ * threads) race and must be ordered by the caller. */
+#define NB_NOALLOC_MAX_BODIES 65536
+int nb_bh_forces_noalloc(const float* pm4, int64_t n, float eps2, float G, float theta,
+                         int32_t leaf_size, int32_t flags, double* force3, int32_t* status,
+                         void* stream);
+
 /* ---- Barnes-Hut debug taps (valid after a BH nb_step or nb_forces) ---- */
 /* codes[n]: 30-bit Morton code of the body in slot i (unsorted order);
  * perm[n]: slot at sorted position k (sorted by (code, slot)). */

@@ -42,15 +42,10 @@
 
 namespace {
 
-constexpr int kBboxBlocks = 1024;
-constexpr int kSortThreads = 256;           // 4 waves
-// elements per lane per pass: 2048-key tiles measured 40 us faster per 4M-body
-// sort than 4096 (more tiles in flight) and than 1024 (larger histograms)
-constexpr int kSortItems = 8;
-constexpr int kSortTile = kSortThreads * kSortItems;  // 2048 elements per block
-constexpr int kScanTile = 2048;             // elements per block in the scans
+// Block and tile sizes shared with the buffer layout are in nb_internal.h. Sort
+// tiles of 2048 keys (8 per lane) measured 40 us faster per 4M-body sort than
+// 4096 (more tiles in flight) and than 1024 (larger histograms).
 constexpr int kScanThreads = 256;
-constexpr int kSpare = 4;  // zero-mass bodies after the sorted body array
 // Traversal groups: each wave of the traversal takes one "wave range" of at
 // most kWaveBodies consecutive sorted bodies. Ranges are cut along the tree: the
 // children of a node holding more than kWaveBodies bodies are packed in order
@@ -781,9 +776,6 @@ __global__ __launch_bounds__(256) void k_tree_emit(const uint32_t* __restrict__ 
 // more than leaf_size; nodes are numbered breadth-first with children in digit
 // order, exactly as the level loop numbers them. The loop then continues from
 // level kTopLevels.
-constexpr int kTopLevels = 4;
-constexpr int kTopCells = (1 + 8 + 64 + 512 + 4096);  // levels 0..4
-constexpr int kTopEntries = kTopCells + kTopLevels + 1;  // plus one end sentinel per level
 
 __host__ __device__ constexpr int top_cell_base(int l) {  // first cell of level l
   return l == 0 ? 0 : top_cell_base(l - 1) + (1 << (3 * (l - 1)));
@@ -1388,24 +1380,52 @@ hipError_t iscan(const int* in, int* out, int* part, const int* level_off, int l
 
 }  // namespace
 
+void nb_bh_carve(nb_ctx* c, char* base, int upto) {
+  nb_bh_state& b = c->bh;
+  const nb_bh_layout L = nb_bh_layout_for(c->n);
+  auto at = [&](int k) -> void* { return k < upto ? base + L.off[k] : nullptr; };
+  b.bbox_part = (float*)at(kBhBbox);
+  b.geom = (float*)at(kBhGeom);
+  b.keys[0] = (uint32_t*)at(kBhKey0);
+  b.keys[1] = (uint32_t*)at(kBhKey1);
+  b.vals[0] = (uint32_t*)at(kBhVal0);
+  b.vals[1] = (uint32_t*)at(kBhVal1);
+  b.codes = (uint32_t*)at(kBhCodes);
+  b.digit_hist = (int*)at(kBhHist);
+  b.iscan_part = (int*)at(kBhIscan);
+  b.pms = (float4*)at(kBhPms);
+  b.prefix = (double4*)at(kBhPrefix);
+  b.scan_part = (double4*)at(kBhScanPart);
+  b.node_com = (float4*)at(kBhCom);
+  b.node_r2 = (float*)at(kBhR2);
+  b.node_link = (int4*)at(kBhLink);
+  b.level_off = (int*)at(kBhLevel);
+  b.child_cnt = (int*)at(kBhChildCnt);
+  b.child_off = (int*)at(kBhChildOff);
+  b.child_bounds = (int*)at(kBhChildBounds);
+  b.counters = (int*)at(kBhCounters);
+  b.top_start = (int*)at(kBhTopStart);
+  b.waves = (int2*)at(kBhWaves);
+  b.inter = (unsigned long long*)at(kBhInter);
+  b.bbox_blocks = kBboxBlocks;
+  b.sort_blocks = (int)L.sort_blocks;
+  b.max_nodes = upto > kBhPms ? L.max_nodes : 0;
+  int dev = 0, cus = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  b.persist_blocks = kListBlocksPerCU * cus;  // resident traversal blocks
+}
+
 int nb_sort_alloc(nb_ctx* c) {
   NB_TRY
   nb_bh_state& b = c->bh;
   if (b.keys[0]) return NB_OK;
-  const int64_t n = c->n;
-  b.bbox_blocks = kBboxBlocks;
-  b.sort_blocks = (int)((n + kSortTile - 1) / kSortTile);
-  const int64_t hist = 256 * (int64_t)b.sort_blocks;
-  const int64_t scan_parts = (std::max<int64_t>(hist, n) + kScanTile - 1) / kScanTile + 2;
-  auto A = [](void** p, size_t bytes) { return hipMalloc(p, bytes < 256 ? 256 : bytes); };
-  if (A((void**)&b.bbox_part, sizeof(float) * 6 * kBboxBlocks) ||
-      A((void**)&b.geom, sizeof(float) * 32) ||
-      A((void**)&b.keys[0], sizeof(uint32_t) * n) || A((void**)&b.keys[1], sizeof(uint32_t) * n) ||
-      A((void**)&b.vals[0], sizeof(uint32_t) * n) || A((void**)&b.vals[1], sizeof(uint32_t) * n) ||
-      A((void**)&b.codes, sizeof(uint32_t) * n) ||
-      A((void**)&b.digit_hist, sizeof(int) * 2 * hist) ||
-      A((void**)&b.iscan_part, sizeof(int) * scan_parts))
+  const nb_bh_layout L = nb_bh_layout_for(c->n);
+  if (hipMalloc(&b.block, (size_t)L.off[kBhPms]) != hipSuccess) {
+    b.block = nullptr;
     return nb_fail(NB_E_OOM, "device allocation failed (Morton sort)");
+  }
+  nb_bh_carve(c, (char*)b.block, kBhPms);
   if (hipMemset(b.geom, 0, sizeof(float) * 32) != hipSuccess)
     return nb_fail(NB_E_HIP, "hipMemset failed (Morton sort)");
   return NB_OK;
@@ -1415,38 +1435,15 @@ int nb_sort_alloc(nb_ctx* c) {
 int nb_bh_alloc(nb_ctx* c) {
   NB_TRY
   nb_bh_state& b = c->bh;
-  const int64_t n = c->n;
-  int64_t bound = 0, w = 1;
-  for (int l = 0; l <= kMaxLevel; ++l) {
-    bound += w < n ? w : n;
-    if (w < n) w *= 8;
-  }
-  b.max_nodes = bound;
-  {
-    int dev = 0, cus = 256;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    b.persist_blocks = kListBlocksPerCU * cus;  // resident traversal blocks
-  }
-  const int rs = nb_sort_alloc(c);
-  if (rs != NB_OK) return rs;
-  auto A = [](void** p, size_t bytes) { return hipMalloc(p, bytes < 256 ? 256 : bytes); };
-  if (A((void**)&b.pms, sizeof(float4) * (n + kSpare)) ||
-      A((void**)&b.prefix, sizeof(double4) * (n + 1)) ||
-      A((void**)&b.scan_part, sizeof(double4) * ((n + kScanTile - 1) / kScanTile + 2)) ||
-      A((void**)&b.node_com, sizeof(float4) * bound) ||
-      A((void**)&b.node_r2, sizeof(float) * bound) ||
-      A((void**)&b.node_link, sizeof(int4) * bound) ||
-      A((void**)&b.level_off, sizeof(int) * (kMaxLevel + 2)) ||
-      A((void**)&b.child_cnt, sizeof(int) * n) || A((void**)&b.child_off, sizeof(int) * n) ||
-      A((void**)&b.child_bounds, sizeof(int) * 8 * n) ||
-      A((void**)&b.counters, sizeof(int) * 16) ||
-      A((void**)&b.top_start, sizeof(int) * kTopEntries) ||
-      A((void**)&b.waves, sizeof(int2) * n) ||
-      A((void**)&b.inter, sizeof(unsigned long long) * 2))
+  const nb_bh_layout L = nb_bh_layout_for(c->n);
+  if (hipMalloc(&b.block, (size_t)L.off[kBhBufCount]) != hipSuccess) {
+    b.block = nullptr;
     return nb_fail(NB_E_OOM, "nb_create: device allocation failed (octree)");
+  }
+  nb_bh_carve(c, (char*)b.block, kBhBufCount);
   // the batched leaf loop may read up to kSpare bodies past the end: zero mass
-  if (hipMemset(b.pms + n, 0, sizeof(float4) * kSpare) != hipSuccess)
+  if (hipMemset(b.pms + c->n, 0, sizeof(float4) * kSpare) != hipSuccess ||
+      hipMemset(b.geom, 0, sizeof(float) * 32) != hipSuccess)
     return nb_fail(NB_E_HIP, "nb_create: hipMemset failed (octree)");
   return NB_OK;
   NB_CATCH("nb_bh_alloc")
@@ -1454,12 +1451,7 @@ int nb_bh_alloc(nb_ctx* c) {
 
 void nb_bh_free(nb_ctx* c) {
   nb_bh_state& b = c->bh;
-  void* ps[] = {b.bbox_part, b.geom,     b.keys[0],  b.keys[1],   b.vals[0],    b.vals[1],
-                b.codes,     b.digit_hist, b.pms,    b.prefix,    b.scan_part,  b.node_com,
-                b.node_link, b.level_off, b.child_cnt, b.child_off, b.iscan_part, b.counters, b.waves,
-                b.inter,     b.top_start, b.child_bounds, b.node_r2};
-  for (void* p : ps)
-    if (p) (void)hipFree(p);
+  if (b.block) (void)hipFree(b.block);
   b = nb_bh_state{};
 }
 

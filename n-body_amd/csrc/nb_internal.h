@@ -67,8 +67,87 @@ constexpr int kPadBodies = 256;
 // Octree geometry: 30-bit Morton codes give at most 10 levels below the root.
 constexpr int kMaxLevel = 10;
 
+// ---- Barnes-Hut buffer layout --------------------------------------------------
+// Every device array of the Barnes-Hut pipeline for n bodies lives in one block:
+// a context allocates it once (nb_bh_alloc), and nb_bh_forces_noalloc carves the
+// same layout out of a static device array. The Morton-sort buffers come first,
+// so a direct-sum context that only reorders (nb_sort_alloc) allocates the
+// prefix [0, off[kBhPms]).
+constexpr int kBboxBlocks = 1024;                     // bbox partial blocks
+constexpr int kSortThreads = 256;                     // 4 waves
+constexpr int kSortItems = 8;                         // keys per lane per radix pass
+constexpr int kSortTile = kSortThreads * kSortItems;  // 2048 keys per block
+constexpr int kScanTile = 2048;                       // elements per block in the scans
+constexpr int kSpare = 4;                             // zero-mass bodies after the sorted bodies
+constexpr int kTopLevels = 4;                         // levels built from per-cell searches
+constexpr int kTopCells = 1 + 8 + 64 + 512 + 4096;    // cells of levels 0..kTopLevels
+constexpr int kTopEntries = kTopCells + kTopLevels + 1;  // plus an end sentinel per level
+
+enum nb_bh_buf {
+  kBhBbox, kBhGeom, kBhKey0, kBhKey1, kBhVal0, kBhVal1, kBhCodes, kBhHist, kBhIscan,
+  kBhPms, kBhPrefix, kBhScanPart, kBhCom, kBhR2, kBhLink, kBhLevel, kBhChildCnt,
+  kBhChildOff, kBhChildBounds, kBhCounters, kBhTopStart, kBhWaves, kBhInter, kBhFlags,
+  kBhBufCount
+};
+
+// Node bound: a level holds at most min(8^l, n) nodes and there are 11 levels.
+constexpr int64_t nb_tree_bound(int64_t n) {
+  int64_t total = 0, w = 1;
+  for (int l = 0; l <= kMaxLevel; ++l) {
+    total += w < n ? w : n;
+    if (w < n) w *= 8;
+  }
+  return total;
+}
+
+struct nb_bh_layout {
+  int64_t off[kBhBufCount + 1] = {};  // byte offsets, 256-aligned; off[kBhBufCount] = total
+  int64_t sort_blocks = 0, max_nodes = 0;
+};
+
+constexpr nb_bh_layout nb_bh_layout_for(int64_t n) {
+  nb_bh_layout L{};
+  const int64_t sort_blocks = (n + kSortTile - 1) / kSortTile;
+  const int64_t hist = 256 * sort_blocks;
+  const int64_t scan_parts = ((hist > n ? hist : n) + kScanTile - 1) / kScanTile + 2;
+  const int64_t bound = nb_tree_bound(n);
+  int64_t bytes[kBhBufCount] = {};
+  bytes[kBhBbox] = 4 * 6 * kBboxBlocks;
+  bytes[kBhGeom] = 4 * 32;
+  bytes[kBhKey0] = bytes[kBhKey1] = bytes[kBhVal0] = bytes[kBhVal1] = bytes[kBhCodes] = 4 * n;
+  bytes[kBhHist] = 4 * 2 * hist;
+  bytes[kBhIscan] = 4 * scan_parts;
+  bytes[kBhPms] = 16 * (n + kSpare);
+  bytes[kBhPrefix] = 32 * (n + 1);
+  bytes[kBhScanPart] = 32 * ((n + kScanTile - 1) / kScanTile + 2);
+  bytes[kBhCom] = 16 * bound;
+  bytes[kBhR2] = 4 * bound;
+  bytes[kBhLink] = 16 * bound;
+  bytes[kBhLevel] = 4 * (kMaxLevel + 2);
+  bytes[kBhChildCnt] = bytes[kBhChildOff] = 4 * n;
+  bytes[kBhChildBounds] = 4 * 8 * n;
+  bytes[kBhCounters] = 4 * 16;
+  bytes[kBhTopStart] = 4 * kTopEntries;
+  bytes[kBhWaves] = 8 * n;
+  bytes[kBhInter] = 8 * 2;
+  bytes[kBhFlags] = 4 * 4;
+  int64_t at = 0;
+  for (int k = 0; k < kBhBufCount; ++k) {
+    L.off[k] = at;
+    at += (bytes[k] + 255) / 256 * 256;
+  }
+  L.off[kBhBufCount] = at;
+  L.sort_blocks = sort_blocks;
+  L.max_nodes = bound;
+  return L;
+}
+
+// Largest system nb_bh_forces_noalloc accepts (its static scratch is sized for it).
+constexpr int64_t kNoAllocMaxBodies = NB_NOALLOC_MAX_BODIES;
+
 // Device arrays of the Barnes-Hut pipeline (sized once at nb_create).
 struct nb_bh_state {
+  void* block = nullptr;        // owning allocation the arrays below are carved from (or null)
   int64_t max_nodes = 0;
   // bbox: per-block partial min/max and the final (min xyz, max xyz, side2[11])
   float* bbox_part = nullptr;   // [2 * 3 * bbox_blocks]
@@ -164,7 +243,11 @@ hipError_t nb_launch_energy(nb_ctx* c, double* ekin_p, double* epot_p);
 
 // Barnes-Hut pipeline
 int nb_sort_alloc(nb_ctx* c);  // bbox, Morton and radix-sort buffers only
-int nb_bh_alloc(nb_ctx* c);    // everything (calls nb_sort_alloc)
+int nb_bh_alloc(nb_ctx* c);    // everything, one allocation
+// Point every Barnes-Hut array of c (n bodies) into `base`, laid out by
+// nb_bh_layout_for(n); `upto` = kBhBufCount for all of them, kBhPms for the
+// sort buffers only. Sets max_nodes, sort_blocks, bbox_blocks, persist_blocks.
+void nb_bh_carve(nb_ctx* c, char* base, int upto);
 hipError_t nb_morton_sort(nb_ctx* c);  // bbox, Morton codes, stable sort: perm in bh.vals[0]
 // slots of pos/vel/acc/force/mass take the bodies of bh.vals[0] (staging in c->tmp)
 hipError_t nb_launch_permute_state(nb_ctx* c);

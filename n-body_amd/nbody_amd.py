@@ -36,6 +36,7 @@ F_NO_GRAPH = 0x1
 F_BH_PER_BODY = 0x4
 F_BH_AXIS_BOX = 0x8
 BH_WAVE_BODIES = 32  # NB_BH_WAVE_BODIES
+NOALLOC_MAX_BODIES = 65536  # NB_NOALLOC_MAX_BODIES
 
 
 class NBodyError(RuntimeError):
@@ -191,6 +192,8 @@ _SIGS = {
     "nb_load_snapshot": (C.c_int, [_P, C.c_char_p]),
     "nb_debug_morton": (C.c_int, [_P, _P, _P]),
     "nb_debug_tree": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int64), _P]),
+    "nb_bh_forces_noalloc": (C.c_int, [_P, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_int32,
+                                       C.c_int32, _P, _P, _P]),
     "nb_reorder": (C.c_int, [_P]),
     "nb_render_view": (C.c_int, [_P, _P]),
     "nb_get_order": (C.c_int, [_P, _P]),

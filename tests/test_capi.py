@@ -65,6 +65,16 @@ def test_argument_errors(nb):
     cfg = nb.Config(n=715827883, eps2=1.0, dt=0.01)
     lib.nb_create(ctypes.byref(ctx), ctypes.byref(cfg))
     assert b"715827882" in lib.nb_last_error()
+    # the allocation-free pass checks its arguments before any device call
+    dummy = ctypes.c_void_p(16)
+    for args in ((dummy, nb.NOALLOC_MAX_BODIES + 1, 1.0, 1.0, 0.5, 8, 0, dummy, None, None),
+                 (dummy, 0, 1.0, 1.0, 0.5, 8, 0, dummy, None, None),
+                 (None, 100, 1.0, 1.0, 0.5, 8, 0, dummy, None, None),
+                 (dummy, 100, -1.0, 1.0, 0.5, 8, 0, dummy, None, None),
+                 (dummy, 100, 1.0, 1.0, 0.5, 65, 0, dummy, None, None),
+                 (dummy, 100, 1.0, 1.0, 0.5, 8, nb.F_NO_GRAPH, dummy, None, None)):
+        assert lib.nb_bh_forces_noalloc(*args) == nb.NB_E_ARG, args
+        assert lib.nb_last_error()
 
 
 def test_initial_conditions_are_seeded(nb):
