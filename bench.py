@@ -13,21 +13,30 @@ Also reported on the same line:
   cpu_baseline  the oracle's v2 restatement (OpenMP, the reference's flags) on a
                 bounded sample of the same workload on this host;
   bh_4m         Barnes-Hut step time on two colliding exponential discs,
-                N = 2^22, theta = 0.5 (skip with --no-bh), with the force error on
-                seeded targets and the FP64 energy drift over the run (both
-                outside the timed steps); its roofline object
-                prices the traversal kernel at 20 flop per interaction counted on
-                the first timed step, over the mean traversal launch time;
+                N = 2^22, theta = 0.5 (skip with --no-bh), leaf size tuned on
+                the discs themselves (nb_tune_leaf), with the force error on
+                4096 seeded targets and the FP64 energy drift over the run
+                (both outside the timed steps); its roofline object prices
+                the traversal kernel at 20 flop per interaction performed on
+                the first timed step, over the mean traversal launch time, and
+                roofline_per_body_count at the interactions the same opening
+                test needs body by body (group_excess is their ratio);
                 ms_per_step_after_reorder times the same steps once more after
                 one Morton reorder of the resident state (nb_reorder: layout
                 only, same work, bitwise the same forces);
   batch_c4      512 independent Plummer clusters (N_c log-uniform in
                 [256, 32768]) in one persistent launch per step (--no-batch),
-                with the same roofline object for the batched force kernel;
-  serve         the Poisson job stream (50 jobs/s) through the continuous-
+                with the same roofline object for the batched force kernel and
+                the v2 restatement timed cluster by cluster as its CPU baseline;
+  c1            the reference's CPU-runnable case (N = 4096, 100 steps) on the
+                GPU, and on the host's cores with the v1 and v2 restatements;
+  serve         the 120-s Poisson job stream (50 jobs/s) through the continuous-
                 batching server (look-ahead 16 and 4096) and one job at a time,
-                S seconds each (--serve S, default 20; 0 skips it): jobs and
-                body-steps finished per second and p50/p99 job latency.
+                the first S seconds each (--serve S, default 20, labelled a
+                reduced window below 120; 0 skips it): jobs and body-steps
+                finished per second and p50/p99 job latency.
+Force errors are measured against nb_direct_f64, the FP64 direct sum on the
+device (the GPU tests pin it to the CPU oracle).
 """
 from __future__ import annotations
 
@@ -156,25 +165,14 @@ def cpu_baseline_bh(pos, mass, budget_s: float):
     }
 
 
-def force_errors(pos, mass, force, targets):
-    """Relative force error on `targets` against an independent FP64 direct sum
-    (numpy, G = 1) over all sources, from the FP32 positions the force pass sees.
+def force_errors(sim, force, targets):
+    """Relative force error on `targets` against the FP64 direct sum over all
+    sources (nb_direct_f64: on the device, from the same FP32 positions the
+    force pass reads; pinned against the CPU oracle by the GPU tests).
     Outside every timed region."""
-    from concurrent.futures import ThreadPoolExecutor
-
-    p = pos.astype(np.float32).astype(np.float64)
-    mm = mass.astype(np.float64)
-
-    def one(t):
-        d = p - p[t]
-        r2 = np.einsum("ij,ij->i", d, d) + EPS2
-        w = mm / (r2 * np.sqrt(r2))
-        w[t] = 0.0
-        ref = w @ d
-        return np.linalg.norm(force[t] - ref) / np.linalg.norm(ref)
-
-    with ThreadPoolExecutor(min(16, host_threads())) as ex:
-        e = np.array(list(ex.map(one, [int(t) for t in targets])))
+    ref = sim.direct_f64(targets)
+    f = force[np.asarray(targets)]
+    e = np.linalg.norm(f - ref, axis=1) / np.linalg.norm(ref, axis=1)
     return {"targets": len(targets), "p50": float(np.median(e)), "p99": float(np.percentile(e, 99)),
             "max": float(e.max())}
 
@@ -192,9 +190,14 @@ def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0, accuracy: boo
     t0 = time.perf_counter()
     pos, vel, m = nb.ic_discs(2024, n)
     log(f"bh: discs generated ({time.perf_counter() - t0:.1f} s)")
-    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH) as sim:  # tuned leaf size
+    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH) as sim:
         sim.upload(pos, vel, m)
+        tuned = sim.tune_leaf(2)  # on these discs, not on a stand-in system
+        log(f"bh: leaf size {tuned['leaf_size']} tuned on the discs ({tuned['bh_ms']:.2f} ms force pass)")
         inter = sim.count_interactions()
+        inter_pb = sim.count_interactions_per_body()
+        log(f"bh: {inter:.4e} interactions performed, {inter_pb:.4e} needed body by body "
+            f"(ratio {inter / inter_pb:.3f})")
         e0 = sum(sim.energy()[:2]) if accuracy else None  # FP64 direct potential, untimed
         sim.step(warmup)
         sim.sync()
@@ -205,12 +208,11 @@ def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0, accuracy: boo
         wall = time.perf_counter() - t0
         t = sim.timing()
         acc = None
-        if accuracy:  # forces at the final state, 64 seeded targets
+        if accuracy:  # forces at the final state, 4096 seeded targets
             sim.set_timing(False)
-            pf, _, _ = sim.download_f64()
             f = sim.forces_f64()
-            tg = np.random.default_rng(7).choice(n, 64, replace=False)
-            fe = force_errors(pf, m, f, tg)
+            tg = np.random.default_rng(7).choice(n, 4096, replace=False)
+            fe = force_errors(sim, f, tg)
             e1 = sum(sim.energy()[:2])
             acc = {"force_error_vs_fp64": fe, "energy_drift": abs(e1 - e0) / abs(e0),
                    "energy_steps": warmup + steps}
@@ -228,6 +230,7 @@ def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0, accuracy: boo
         reordered_ms = 1e3 * (time.perf_counter() - t0) / steps
         log(f"bh: {reordered_ms:.2f} ms/step after one Morton reorder")
     ms = 1e3 * wall / steps
+    trav_ms = t["force_ms"] / max(1, t["force_calls"])
     cpu = None
     if cpu_budget > 0:
         log("bh: cpu baseline (oracle v6 restatement)")
@@ -235,15 +238,18 @@ def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0, accuracy: boo
     return {
         "workload": "Barnes-Hut N=4194304, two exponential discs (Rd=30, h=2, 300 apart, "
                     "closing 0.5/step), theta=0.5, eps2=40, dt=0.0035",
-        "leaf_size": nb.get_tuning(n)["leaf_size"],
+        "leaf_size": tuned["leaf_size"],
         "steps": steps,
         "ms_per_step": ms,
         "ms_per_step_after_reorder": reordered_ms,
         "build_ms": t["build_ms"] / max(1, t["build_calls"]),
-        "traverse_ms": t["force_ms"] / max(1, t["force_calls"]),
+        "traverse_ms": trav_ms,
         "interactions_first_step": inter,
-        "interactions_per_s": inter / (t["force_ms"] / max(1, t["force_calls"]) / 1e3),
-        "roofline": valu_roofline("k_traverse_list", inter, t["force_ms"] / max(1, t["force_calls"])),
+        "interactions_per_body_test": inter_pb,
+        "group_excess": inter / inter_pb,
+        "interactions_per_s": inter / (trav_ms / 1e3),
+        "roofline": valu_roofline("k_traverse_list", inter, trav_ms),
+        "roofline_per_body_count": valu_roofline("k_traverse_list", inter_pb, trav_ms),
         "accuracy": acc,
         "cpu_baseline": cpu,
     }
@@ -262,7 +268,50 @@ def c4_clusters(nb, count=512, seed=4000):
     return out
 
 
-def bench_batch(nb, steps: int, warmup: int, energy: bool, tune: bool = True):
+def cpu_baseline_batch(clusters, budget_s: float):
+    """v2 (ParticleSolverCPUParallel.cpp:12-40) force pass of the ensemble on this
+    host's cores, one cluster after another as a CPU would run them: seeded
+    clusters are timed in a random order until the budget is spent, and the
+    rate over their interactions is extrapolated to the whole ensemble."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as ora
+
+    th = host_threads()
+    lib = ora.load(fast=True)
+    order = np.random.default_rng(17).permutation(len(clusters))
+    done_inter, el, used = 0.0, 0.0, 0
+    for c in order:
+        pos, _, m, eps2, _ = clusters[c]
+        k = len(m)
+        p4 = np.zeros((k, 4), np.float32)
+        p4[:, :3] = pos
+        m4 = np.zeros((k, 4), np.float32)
+        m4[:, 0] = m
+        out = np.zeros((k, 4), np.float32)
+        t0 = time.perf_counter()
+        lib.or_pp_forces(k, ora._p(p4), ora._p(m4), eps2, 1.0, ora._p(out), 0, k, th)
+        el += time.perf_counter() - t0
+        done_inter += float(k) * (k - 1)
+        used += 1
+        if el > budget_s:
+            break
+    total = sum(float(len(c[2])) * (len(c[2]) - 1) for c in clusters)
+    rate = done_inter / el
+    return {
+        "value": rate,
+        "unit": "interactions/s",
+        "ms_per_step": 1e3 * total / rate,
+        "cores": th,
+        "kind": "port",
+        "cpu": host_cpu(),
+        "sample": f"v2 force pass (OpenMP static over each cluster's bodies, -O3 -ffast-math) of "
+                  f"{used} of the {len(clusters)} clusters in seeded random order "
+                  f"({done_inter:.3e} of {total:.3e} interactions per step, {el:.1f} s), "
+                  f"extrapolated to the whole ensemble",
+    }
+
+
+def bench_batch(nb, steps: int, warmup: int, energy: bool, tune: bool = True, cpu_budget: float = 0.0):
     t0 = time.perf_counter()
     cl = c4_clusters(nb)
     log(f"batch: 512 clusters generated ({time.perf_counter() - t0:.1f} s)")
@@ -303,7 +352,45 @@ def bench_batch(nb, steps: int, warmup: int, energy: bool, tune: bool = True):
             res["energy_drift_p50"] = float(np.median(drift))
             res["energy_drift_max"] = float(drift.max())
             res["energy_steps"] = steps + warmup
+    res["cpu_baseline"] = cpu_baseline_batch(cl, cpu_budget) if cpu_budget > 0 else None
     return res
+
+
+def bench_c1(nb, cpu: bool):
+    """The reference's own CPU-runnable case: N = 4096 uniform ball (R = 100, m = 1,
+    v = 0), eps2 = 40, dt = 0.0035, 100 steps of the lagged KDK. The GPU runs all
+    100 steps; the CPU restatements of v1 (sequential) and v2 (OpenMP) run them too."""
+    n, steps = 4096, 100
+    pos, vel, m = nb.ic_ball(1, n, 100.0, 1.0)
+    with nb.Simulation(n, EPS2, DT) as sim:
+        sim.upload(pos, vel, m)
+        sim.step(2)
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        sim.sync()
+        wall = time.perf_counter() - t0
+    out = {"workload": "direct sum N=4096 uniform ball R=100, m=1, eps2=40, dt=0.0035, 100 steps",
+           "ms_per_step": 1e3 * wall / steps, "interactions_per_s": n * (n - 1) * steps / wall}
+    if cpu:
+        sys.path.insert(0, str(ROOT / "oracle"))
+        import oracle as ora
+
+        p4 = np.zeros((n, 4), np.float32)
+        p4[:, :3] = pos
+        m4 = np.zeros((n, 4), np.float32)
+        m4[:, 0] = m
+        th = host_threads()
+        for name, threads, k in (("v1", 1, 10), ("v2", th, steps)):
+            p, v, a = p4.copy(), np.zeros_like(p4), np.zeros_like(p4)
+            t0 = time.perf_counter()
+            for _ in range(k):
+                ora.pp_step(p, v, a, m4, EPS2, DT, nthreads=threads, fast=True)
+            el = time.perf_counter() - t0
+            out[f"cpu_{name}"] = {"ms_per_step": 1e3 * el / k, "interactions_per_s": n * (n - 1) * k / el,
+                                  "cores": threads, "kind": "port", "cpu": host_cpu(),
+                                  "sample": f"{name} restatement, {k} of the 100 steps"}
+    return out
 
 
 def poisson_jobs(seed: int, rate: float, duration: float):
@@ -420,7 +507,10 @@ def bench_serve(nb, rate: float, duration: float, limit: float, lookaheads=(16, 
     jobs = poisson_jobs(5050, rate, duration)
     out = {"workload": f"Poisson stream {rate:g} jobs/s for {duration:g} s (seeded), N log-uniform "
                        f"[1024, 2^20], direct below 65536 else Barnes-Hut theta=0.5, steps U[10,500], "
-                       f"Plummer or ball, eps2=40, dt=0.0035; every mode measured for {limit:g} s",
+                       f"Plummer or ball, eps2=40, dt=0.0035",
+           "window_s": limit,
+           "window": "the full stream" if limit >= duration else
+                     f"reduced: the first {limit:g} s of the {duration:g}-s stream, per mode",
            "offered_jobs": len(jobs),
            "offered_body_steps": int(sum(j["n"] * j["steps"] for j in jobs if j["t"] <= limit))}
     ic_threads = 8
@@ -522,8 +612,6 @@ def main():
     if not args.no_tune:
         t0 = time.perf_counter()
         tuning = nb.autotune(n, nb.TUNE_DIRECT, 3)
-        if not args.no_bh:
-            tuning["bh_4m"] = nb.autotune(1 << 22, nb.TUNE_BH, 2)
         log(f"autotune ({time.perf_counter() - t0:.1f} s): {tuning}")
     pos, vel, m = nb.ic_plummer(2018, n, 50.0, 1e4)
     accuracy = None
@@ -542,10 +630,9 @@ def main():
         if not args.no_accuracy:  # outside the timed region
             sim.set_timing(False)
             e1 = sum(sim.energy()[:2])
-            pf, _, _ = sim.download_f64()
             f = sim.forces_f64()
-            tg = np.random.default_rng(5).choice(n, 512, replace=False)
-            accuracy = {"force_error_vs_fp64": force_errors(pf, m, f, tg),
+            tg = np.random.default_rng(5).choice(n, 4096, replace=False)
+            accuracy = {"force_error_vs_fp64": force_errors(sim, f, tg),
                         "energy_drift": abs(e1 - e0) / abs(e0),
                         "energy_steps": args.warmup + args.steps}
             log(f"direct: force error vs FP64 p50 {accuracy['force_error_vs_fp64']['p50']:.2e} "
@@ -599,10 +686,13 @@ def main():
         log(f"bh: {result['bh_4m']['ms_per_step']:.2f} ms/step")
     if not args.no_batch:
         log("batch: C4 ensemble")
-        result["batch_c4"] = bench_batch(nb, args.batch_steps, 2, not args.no_batch_energy, not args.no_tune)
+        result["batch_c4"] = bench_batch(nb, args.batch_steps, 2, not args.no_batch_energy, not args.no_tune,
+                                         0.0 if args.no_cpu else args.cpu_budget)
         log(f"batch: {result['batch_c4']['ms_per_step']:.2f} ms/step")
+    log("c1: N=4096, 100 steps")
+    result["c1"] = bench_c1(nb, not args.no_cpu)
     if args.serve > 0:
-        result["serve"] = bench_serve(nb, args.serve_rate, args.serve, args.serve)
+        result["serve"] = bench_serve(nb, args.serve_rate, 120.0, args.serve)
     if not args.no_cpu:
         log("cpu baseline (oracle v2 restatement)")
         result["cpu_baseline"] = cpu_baseline(pos.astype(np.float32), m, args.cpu_budget)

@@ -227,6 +227,16 @@ int nb_get_timing(nb_ctx* ctx, nb_timing* out);
 /* BH: count body-cell + body-body interactions at the current state (diagnostic
  * traversal, not on the step path). */
 int nb_count_interactions(nb_ctx* ctx, int64_t* interactions);
+/* BH: the interactions the default traversal's opening test needs when every
+ * body walks the same tree on its own (the floor of the group traversal's work;
+ * the group shares decisions among up to NB_BH_WAVE_BODIES bodies and does more).
+ * Direct sum: n (n - 1). Diagnostic, not on the step path. */
+int nb_count_interactions_per_body(nb_ctx* ctx, int64_t* interactions);
+/* FP64 direct-sum accelerations of the bodies targets[0..k) (caller indices)
+ * from the context's FP32 (x, y, z, m) view widened to FP64, G applied:
+ * acc3 = double[3k], interleaved. The error reference for both solvers'
+ * FP32 forces at any n (the CPU oracle pins it). Synchronises. */
+int nb_direct_f64(nb_ctx* ctx, const int64_t* targets, int64_t k, double* acc3);
 
 /* ---- batched ensembles ---------------------------------------------------
  * Many independent particle systems ("clusters") of uneven size, each with its
@@ -363,6 +373,11 @@ int nb_autotune(int64_t n, int32_t what, int32_t reps, nb_tuning* out);
  * particle state is not changed) and keep the fastest for later steps.
  * out->direct_ms = its force pass, out->n = bodies in the batch. */
 int nb_batch_tune(nb_batch* b, int32_t reps, nb_tuning* out);
+/* Barnes-Hut leaf size tuned on the workload itself: times leaf_size in
+ * {1,2,4,8,16,32} with reps force passes each on the context's current bodies
+ * (the particle state is not changed) and keeps the fastest for the context's
+ * later steps. out->leaf_size and out->bh_ms receive the choice. */
+int nb_tune_leaf(nb_ctx* ctx, int32_t reps, nb_tuning* out);
 /* The defaults nb_create would use for n bodies (tuned or built in). */
 int nb_get_tuning(int64_t n, nb_tuning* out);
 

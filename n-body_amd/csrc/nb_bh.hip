@@ -38,6 +38,7 @@
 #include <cfloat>
 #include <vector>
 
+#include "nb_force_core.h"
 #include "nb_internal.h"
 
 namespace {
@@ -993,7 +994,9 @@ __global__ __launch_bounds__(256) void k_tree_com(const int4* __restrict__ link,
 }
 
 // ---- 8. traversal ---------------------------------------------------------------------------
-template <bool kCount>
+// kModified: open with the default traversal's radius (r2[i] < d^2) instead of
+// the reference's test; used only to count what that test needs per body.
+template <bool kCount, bool kModified = false>
 __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ pms,
                                                   const uint32_t* __restrict__ perm, int n,
                                                   const float4* __restrict__ com,
@@ -1004,7 +1007,8 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ pms
                                                   float eps2, double G,
                                                   double* __restrict__ force,
                                                   unsigned long long* __restrict__ inter,
-                                                  int* __restrict__ flags) {
+                                                  int* __restrict__ flags,
+                                                  const float* __restrict__ r2 = nullptr) {
   __shared__ float side2[kMaxLevel + 1];
   if (threadIdx.x <= kMaxLevel) side2[threadIdx.x] = geom[6 + threadIdx.x];
   __syncthreads();
@@ -1026,7 +1030,7 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ pms
       const int4 nd = link[i];
       const float dx = c.x - me.x, dy = c.y - me.y, dz = c.z - me.z;
       const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
-      if (side2[link_level(nd.w)] < theta2 * d2) {
+      if (kModified ? r2[i] < d2 : side2[link_level(nd.w)] < theta2 * d2) {
         if (c.w > 0.f && d2 > 0.f) {
           const float inv = __builtin_amdgcn_rsqf(d2 + eps2);
           const float s = (c.w * inv) * (inv * inv);
@@ -1057,7 +1061,7 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ pms
       }
     }
     const uint32_t o = perm[t];
-    if (o < (uint32_t)n) {
+    if (!kModified && o < (uint32_t)n) {
       force[o] = G * (double)ax;
       force[n + o] = G * (double)ay;
       force[2 * n + o] = G * (double)az;
@@ -1112,12 +1116,17 @@ constexpr uint32_t kCellBit = 0x80000000u;    // list entry refers to a cell, no
 // slice-major (entry j at (j % S) * (64 / S) + j / S), so a slice reads its
 // entries from consecutive addresses: four reads per step share one base
 // address with immediate offsets, and the trip count is wave-uniform.
+// Compensated accumulation: each flush sums its (at most kListCap / 2 per lane
+// at two slices) terms in plain FP32 and folds the partial sums into the
+// running (sum, error) pairs with TwoSum, as the direct sum folds every 256
+// sources.
 template <bool kCount, bool kGuard>
 __device__ __forceinline__ void list_flush(const uint32_t* list, int lc, float4* stage, int lane,
                                            int slice, int log2S, const float4* __restrict__ pms,
                                            const float4* __restrict__ com, float4 me, float eps2_s,
-                                           float& ax, float& ay, float& az,
-                                           unsigned long long& cnt) {
+                                           float& sx, float& sy, float& sz, float& cx, float& cy,
+                                           float& cz, unsigned long long& cnt) {
+  float ax = 0.f, ay = 0.f, az = 0.f;
   float eps2 = eps2_s;  // a VGPR copy: all-VGPR operand forms issue faster
   asm volatile("" : "+v"(eps2));
   const int S = 1 << log2S;
@@ -1161,6 +1170,9 @@ __device__ __forceinline__ void list_flush(const uint32_t* list, int lc, float4*
     }
     __builtin_amdgcn_wave_barrier();
   }
+  nbcore::two_sum_fold(sx, cx, ax);
+  nbcore::two_sum_fold(sy, cy, ay);
+  nbcore::two_sum_fold(sz, cz, az);
 }
 
 // Shared LDS of one traversing wave.
@@ -1178,7 +1190,7 @@ __device__ void traverse_range(const WaveLds& L, int lane, int P, float4 me, int
                                const float4* __restrict__ pms, const float4* __restrict__ com,
                                const int4* __restrict__ link, const float* __restrict__ r2,
                                float eps2, int* __restrict__ flags, float& ax, float& ay, float& az,
-                               unsigned long long& cnt) {
+                               float& cx, float& cy, float& cz, unsigned long long& cnt) {
   const uint64_t lt = (uint64_t(1) << lane) - 1;
   const int log2S = 6 - (31 - __clz(P)), slice = lane / P;  // S = 64 / P
   float lx = me.x, ly = me.y, lz = me.z, hx = me.x, hy = me.y, hz = me.z;
@@ -1191,12 +1203,22 @@ __device__ void traverse_range(const WaveLds& L, int lane, int P, float4 me, int
     hy = fmaxf(hy, __shfl_xor(hy, off));
     hz = fmaxf(hz, __shfl_xor(hz, off));
   }
+  // the box is the same in every lane: keep it in scalar registers
+  auto uniform = [](float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+  };
+  lx = uniform(lx);
+  ly = uniform(ly);
+  lz = uniform(lz);
+  hx = uniform(hx);
+  hy = uniform(hy);
+  hz = uniform(hz);
   int2* stack = L.stack;
   uint32_t* list = L.list;
 #define NB_FLUSH()                                                                              \
   do {                                                                                          \
     list_flush<kCount, kGuard>(list, lc, L.stage, lane, slice, log2S, pms, com, me, eps2, ax, ay, \
-                               az, cnt);                                                       \
+                               az, cx, cy, cz, cnt);                                           \
     lc = 0;                                                                                     \
   } while (0)
   // a virtual parent whose only child is the root
@@ -1317,23 +1339,26 @@ __global__ __launch_bounds__(64 * kListWaves, kCount ? 5 : kListBlocksPerCU) voi
     const bool valid = slot < range.y;
     const int t = min(range.x + (valid ? slot : 0), n - 1);
     const float4 me = pms[t];
-    float ax = 0.f, ay = 0.f, az = 0.f;
+    float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
     unsigned long long c = 0;
     traverse_range<kCount, kGuard>(L, lane, P, me, n, pms, com, link, r2, eps2, flags, ax,
-                                   ay, az, c);
-    // add the slices' partial sums (butterfly over the slice bits, fixed order)
+                                   ay, az, cx, cy, cz, c);
+    // each slice's compensated sum in FP64, then the slices added (butterfly
+    // over the slice bits, fixed order)
+    double fx = (double)ax + (double)cx, fy = (double)ay + (double)cy,
+           fz = (double)az + (double)cz;
     for (int off = P; off < 64; off <<= 1) {
-      ax += __shfl_xor(ax, off);
-      ay += __shfl_xor(ay, off);
-      az += __shfl_xor(az, off);
+      fx += __shfl_xor(fx, off);
+      fy += __shfl_xor(fy, off);
+      fz += __shfl_xor(fz, off);
     }
     if (valid) cnt += c;
     if (valid && lane < P) {
       const uint32_t o = perm[t];
       if (o < (uint32_t)n) {
-        force[o] = G * (double)ax;
-        force[n + o] = G * (double)ay;
-        force[2 * n + o] = G * (double)az;
+        force[o] = G * fx;
+        force[n + o] = G * fy;
+        force[2 * n + o] = G * fz;
       }
     }
   }
@@ -1578,6 +1603,16 @@ hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions) {
   else
     hipLaunchKernelGGL((k_traverse<false>), dim3(g256), dim3(256), 0, c->stream, NB_TRAVERSE_ARGS);
 #undef NB_TRAVERSE_ARGS
+  return hipGetLastError();
+}
+
+hipError_t nb_bh_count_per_body(nb_ctx* c) {
+  nb_bh_state& b = c->bh;
+  const int n = c->n;
+  hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, c->stream, b.inter);
+  hipLaunchKernelGGL((k_traverse<true, true>), dim3((n + 255) / 256), dim3(256), 0, c->stream, b.pms,
+                     b.vals[0], n, b.node_com, b.node_link, b.level_off, b.max_nodes, b.geom,
+                     c->theta2, c->cfg.eps2, c->G, c->force, b.inter, c->flags, b.node_r2);
   return hipGetLastError();
 }
 

@@ -668,6 +668,61 @@ int nb_count_interactions(nb_ctx* c, int64_t* interactions) {
   NB_CATCH("nb_count_interactions")
 }
 
+int nb_count_interactions_per_body(nb_ctx* c, int64_t* interactions) {
+  NB_TRY
+  NB_API_SHARED;
+  if (!c || !interactions) return nb_fail(NB_E_ARG, "nb_count_interactions_per_body: null argument");
+  if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_count_interactions_per_body: nothing uploaded");
+  if (c->cfg.solver != NB_SOLVER_BH) {
+    *interactions = (int64_t)c->n * (c->n - 1);
+    return NB_OK;
+  }
+  NB_HIP(nb_bh_build(c));
+  NB_HIP(nb_bh_count_per_body(c));
+  unsigned long long h = 0;
+  NB_HIP(hipMemcpyAsync(&h, c->bh.inter, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  NB_HIP(hipStreamSynchronize(c->stream));
+  *interactions = (int64_t)h;
+  c->tree_valid = true;
+  return check_flags(c);
+  NB_CATCH("nb_count_interactions_per_body")
+}
+
+int nb_direct_f64(nb_ctx* c, const int64_t* targets, int64_t k, double* acc3) {
+  NB_TRY
+  NB_RANGE("nb_direct_f64");
+  NB_API_SHARED;
+  if (!c || (k > 0 && (!targets || !acc3))) return nb_fail(NB_E_ARG, "nb_direct_f64: null argument");
+  if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_direct_f64: nothing uploaded");
+  if (k < 0 || k > c->n) return nb_fail(NB_E_ARG, "nb_direct_f64: k must be in [0, n]");
+  if (k == 0) return NB_OK;
+  // caller indices -> device slots (the inverse of the reorder map)
+  std::vector<int32_t> slot_of;
+  if (!c->order.empty()) {
+    slot_of.resize(c->n);
+    for (int s = 0; s < c->n; ++s) slot_of[c->order[s]] = s;
+  }
+  std::vector<int32_t> slots(k);
+  for (int64_t t = 0; t < k; ++t) {
+    if (targets[t] < 0 || targets[t] >= c->n) return nb_fail(NB_E_ARG, "nb_direct_f64: target out of range");
+    slots[t] = slot_of.empty() ? (int32_t)targets[t] : slot_of[targets[t]];
+  }
+  void* d = nullptr;
+  const size_t bytes = (size_t)k * (sizeof(int32_t) + 3 * sizeof(double)) + 256;
+  if (hipMalloc(&d, bytes) != hipSuccess) return nb_fail(NB_E_OOM, "nb_direct_f64: device allocation failed");
+  int* d_slots = static_cast<int*>(d);
+  double* d_acc = reinterpret_cast<double*>(static_cast<char*>(d) + ((size_t)k * sizeof(int32_t) + 255) / 256 * 256);
+  hipError_t e = hipMemcpyAsync(d_slots, slots.data(), sizeof(int32_t) * k, hipMemcpyHostToDevice, c->stream);
+  if (e == hipSuccess) e = nb_launch_ref_f64(c, d_slots, (int)k, d_acc);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(acc3, d_acc, sizeof(double) * 3 * k, hipMemcpyDeviceToHost, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return nb_fail(NB_E_HIP, std::string("nb_direct_f64: ") + hipGetErrorString(e));
+  return NB_OK;
+  NB_CATCH("nb_direct_f64")
+}
+
 int nb_device_info(char* buf, int32_t cap) {
   NB_TRY
   if (!buf || cap < 1) return nb_fail(NB_E_ARG, "nb_device_info: bad buffer");

@@ -203,7 +203,48 @@ __global__ __launch_bounds__(256) void k_energy_pot(int n, const double* __restr
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// FP64 direct-sum acceleration of selected bodies: the error reference for the
+// FP32 solvers, computed on the device from the same FP32 (x, y, z, m) view the
+// force passes read, widened to FP64. One workgroup per target; thread t adds
+// the sources t, t + 256, ... in order, and the 256 partials are added in a
+// fixed tree, so the result does not depend on scheduling.
+__global__ __launch_bounds__(256) void k_ref_f64(const float4* __restrict__ pm, int n,
+                                                 const int* __restrict__ slots, double eps2,
+                                                 double G, double* __restrict__ acc3) {
+  __shared__ double red[3][256];
+  const int i = slots[blockIdx.x];
+  const float4 me = pm[i];
+  const double xi = me.x, yi = me.y, zi = me.z;
+  double ax = 0.0, ay = 0.0, az = 0.0;
+  for (int j = threadIdx.x; j < n; j += 256) {
+    const float4 q = pm[j];
+    const double dx = (double)q.x - xi, dy = (double)q.y - yi, dz = (double)q.z - zi;
+    const double r2 = dx * dx + dy * dy + dz * dz + eps2;
+    const double s = j == i ? 0.0 : (double)q.w / (r2 * sqrt(r2));
+    ax += s * dx;
+    ay += s * dy;
+    az += s * dz;
+  }
+  red[0][threadIdx.x] = ax;
+  red[1][threadIdx.x] = ay;
+  red[2][threadIdx.x] = az;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int a = 0; a < 3; ++a) red[a][threadIdx.x] += red[a][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 3) acc3[3 * blockIdx.x + threadIdx.x] = G * red[threadIdx.x][0];
+}
+
 }  // namespace
+
+hipError_t nb_launch_ref_f64(nb_ctx* c, const int* slots, int k, double* acc3) {
+  if (k <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ref_f64, dim3(k), dim3(256), 0, c->stream, c->pm, c->n, slots,
+                     (double)c->cfg.eps2, c->G, acc3);
+  return hipGetLastError();
+}
 
 hipError_t nb_launch_direct(nb_ctx* c) {
   switch (c->bpl) {

@@ -240,6 +240,8 @@ hipError_t nb_launch_refresh_pm(nb_ctx* c);  // pos, mass -> pm
 hipError_t nb_launch_fill_i32(int* p, int v, int64_t count, hipStream_t s);
 hipError_t nb_launch_render_view(nb_ctx* c, float4* out);  // (x, y, z, |v|) per slot
 hipError_t nb_launch_energy(nb_ctx* c, double* ekin_p, double* epot_p);
+// FP64 direct-sum accelerations of the bodies in slots[0..k) -> acc3[3k]
+hipError_t nb_launch_ref_f64(nb_ctx* c, const int* slots, int k, double* acc3);
 
 // Barnes-Hut pipeline
 int nb_sort_alloc(nb_ctx* c);  // bbox, Morton and radix-sort buffers only
@@ -254,6 +256,9 @@ hipError_t nb_launch_permute_state(nb_ctx* c);
 void nb_bh_free(nb_ctx* c);
 hipError_t nb_bh_build(nb_ctx* c);     // bbox, Morton, sort, permute, prefix, octree
 hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions);  // -> force
+// Per-body traversal with the default (modified) opening test, counting only:
+// the interactions the group traversal's test needs body by body -> bh.inter
+hipError_t nb_bh_count_per_body(nb_ctx* c);
 
 // Launch-shape defaults: fill the zero fields from the autotuner's table, then
 // from the built-in defaults.

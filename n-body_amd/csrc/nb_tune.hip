@@ -149,6 +149,61 @@ extern "C" int nb_autotune(int64_t n, int32_t what, int32_t reps, nb_tuning* out
   NB_CATCH("nb_autotune")
 }
 
+extern "C" int nb_tune_leaf(nb_ctx* c, int32_t reps, nb_tuning* out) {
+  NB_TRY
+  NB_RANGE("nb_tune_leaf");
+  NB_API_SHARED;
+  if (!c) return nb_fail(NB_E_ARG, "nb_tune_leaf: null context");
+  if (c->cfg.solver != NB_SOLVER_BH) return nb_fail(NB_E_ARG, "nb_tune_leaf: not a Barnes-Hut context");
+  if (!c->uploaded) return nb_fail(NB_E_ARG, "nb_tune_leaf: nothing uploaded");
+  if (reps < 1) reps = 3;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return nb_fail(NB_E_HIP, "nb_tune_leaf: event creation failed");
+  }
+  int rc = NB_OK, best_leaf = c->cfg.leaf_size;
+  double best_ms = DBL_MAX;
+  for (int leaf : {1, 2, 4, 8, 16, 32}) {
+    c->cfg.leaf_size = leaf;
+    for (int r = -1; !rc && r < reps; ++r) {  // one warm-up pass
+      if (r == 0) (void)hipEventRecord(e0, c->stream);
+      hipError_t e = nb_bh_build(c);
+      if (e == hipSuccess) e = nb_bh_traverse(c, false);
+      if (e != hipSuccess) rc = nb_fail(NB_E_HIP, std::string("nb_tune_leaf: ") + hipGetErrorString(e));
+    }
+    if (rc) break;
+    (void)hipEventRecord(e1, c->stream);
+    float t = 0.f;
+    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+      rc = nb_fail(NB_E_HIP, "nb_tune_leaf: timing failed");
+      break;
+    }
+    if (t / reps < best_ms) {
+      best_ms = t / reps;
+      best_leaf = leaf;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->cfg.leaf_size = best_leaf;
+  // the captured step graph holds the old leaf size; the tree and forces of the
+  // last timed pass are the last candidate's, so the chosen one runs once more
+  nb_drop_graph(c->stream, c->graph_exec, c->graph);
+  if (rc) return rc;
+  NB_HIP(nb_bh_build(c));
+  NB_HIP(nb_bh_traverse(c, false));
+  NB_HIP(hipStreamSynchronize(c->stream));
+  if (out) {
+    *out = nb_tuning{};
+    out->n = c->n;
+    out->leaf_size = best_leaf;
+    out->bh_ms = best_ms;
+  }
+  return NB_OK;
+  NB_CATCH("nb_tune_leaf")
+}
+
 extern "C" int nb_get_tuning(int64_t n, nb_tuning* out) {
   NB_TRY
   if (!out || n < 1) return nb_fail(NB_E_ARG, "nb_get_tuning: bad argument");

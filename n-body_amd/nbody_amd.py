@@ -200,6 +200,9 @@ _SIGS = {
     "nb_set_timing": (C.c_int, [_P, C.c_int32]),
     "nb_get_timing": (C.c_int, [_P, C.POINTER(Timing)]),
     "nb_count_interactions": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "nb_tune_leaf": (C.c_int, [_P, C.c_int32, C.POINTER(Tuning)]),
+    "nb_count_interactions_per_body": (C.c_int, [_P, C.POINTER(C.c_int64)]),
+    "nb_direct_f64": (C.c_int, [_P, _P, C.c_int64, _P]),
     "nb_device_info": (C.c_int, [C.c_char_p, C.c_int32]),
     "nb_batch_create": (C.c_int, [C.POINTER(_P), C.POINTER(BatchConfig)]),
     "nb_batch_destroy": (None, [_P]),
@@ -546,6 +549,24 @@ class Simulation:
         v = C.c_int64()
         _check(self._lib.nb_count_interactions(self._ctx, C.byref(v)))
         return v.value
+
+    def count_interactions_per_body(self) -> int:
+        v = C.c_int64()
+        _check(self._lib.nb_count_interactions_per_body(self._ctx, C.byref(v)))
+        return v.value
+
+    def direct_f64(self, targets) -> np.ndarray:
+        """FP64 direct-sum accelerations of the given bodies, on the device."""
+        t = np.ascontiguousarray(targets, np.int64)
+        out = np.empty((len(t), 3), np.float64)
+        _check(self._lib.nb_direct_f64(self._ctx, _ptr(t), len(t), _ptr(out)))
+        return out
+
+    def tune_leaf(self, reps: int = 3) -> dict:
+        """Barnes-Hut leaf size timed on this system's own bodies (nb_tune_leaf)."""
+        t = Tuning()
+        _check(self._lib.nb_tune_leaf(self._ctx, reps, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in Tuning._fields_ if k != "pad_"}
 
 
 class Batch:

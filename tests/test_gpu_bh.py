@@ -287,3 +287,46 @@ def test_largest_tested_size_2_24(nb, ora):
     e = rel_err(f[tgt], ref)
     print(f"N=2^24 Plummer vs FP64 (64 targets): p50={np.median(e):.2e} p99={np.percentile(e, 99):.2e}")
     assert np.median(e) < 1e-2
+
+
+def test_fp64_reference_on_device(nb, ora):
+    """nb_direct_f64 (the device FP64 direct sum the bench prices errors with)
+    against the CPU oracle's FP64 direct sum on the same FP32 positions, before
+    and after a Morton reorder (targets are caller indices)."""
+    n = 20000
+    pos, vel, m = nb.ic_plummer(77, n, 50.0, 1e4)
+    tg = np.random.default_rng(8).choice(n, 500, replace=False)
+    ref = ora.direct_f64(pos.astype(np.float32).astype(np.float64), m.astype(np.float64), EPS2,
+                         targets=tg)
+    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH) as sim:
+        sim.upload(pos, vel, m)
+        a = sim.direct_f64(tg)
+        sim.reorder()
+        b = sim.direct_f64(tg)
+    assert rel_err(a, ref).max() < 1e-12
+    assert rel_err(b, ref).max() < 1e-12
+
+
+def test_per_body_count_and_leaf_tuning(nb, ora):
+    n = 30000
+    pos, vel, m = nb.ic_discs(78, n)
+    p4 = pm4(pos, m)
+    mn, mx, ccodes, cperm, tree, com = cpu_build(ora, p4, 8)
+    _, cpb = ora.bh_forces_group(p4[cperm], cperm, mn, mx, tree, com, EPS2, group=1,
+                                 codes_sorted=ccodes[cperm])
+    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH, leaf_size=8) as sim:
+        sim.upload(pos, vel, m)
+        performed = sim.count_interactions()
+        per_body = sim.count_interactions_per_body()
+        print(f"discs N={n} leaf 8: performed {performed}, per body {per_body} (CPU {cpb}), "
+              f"ratio {performed / per_body:.3f}")
+        assert abs(per_body - cpb) <= 1e-4 * cpb
+        assert performed >= per_body
+        t = sim.tune_leaf(2)
+        assert t["leaf_size"] in (1, 2, 4, 8, 16, 32) and t["bh_ms"] > 0
+        f_tuned = sim.forces_f64()
+        sim.step(2)
+        sim.sync()
+    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH, leaf_size=t["leaf_size"]) as ref:
+        ref.upload(pos, vel, m)
+        np.testing.assert_array_equal(ref.forces_f64(), f_tuned)
