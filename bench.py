@@ -449,6 +449,7 @@ def run_stream(nb, jobs, limit_s: float, **server_kw):
             queued += st["state"] == nb.JOB_QUEUED
             running += st["state"] == nb.JOB_RUNNING
             body_steps += j["n"] * st["steps_done"]
+        log(f"serve: loop {s.stats()}")
     return done, body_steps, k, queued, running
 
 
@@ -503,6 +504,78 @@ def run_one_at_a_time(nb, jobs, limit_s: float, ic_threads: int):
     return done, body_steps, min(len(jobs), k_prep), queued, int(k < len(jobs))
 
 
+def job_cost_model(nb):
+    """Device seconds of one job run alone, from timed runs on this GPU:
+    cost = steps * step(N) + setup(N), with step(N) and setup(N) (context
+    creation, upload, download, teardown) interpolated in log N between timed
+    sizes of each solver."""
+    pts = {}
+    for solver, sizes, k in ((nb.SOLVER_DIRECT, (1024, 4096, 16384, 65535), 32),
+                             (nb.SOLVER_BH, (65536, 262144, 1048576), 16)):
+        rows = []
+        for n in sizes:
+            pos, vel, m = nb.ic_plummer(99, n, 50.0, 1e4)
+            t0 = time.perf_counter()
+            with nb.Simulation(n, EPS2, DT, solver=solver) as sim:
+                sim.upload(pos, vel, m)
+                sim.step(2)
+                sim.sync()
+                t1 = time.perf_counter()
+                sim.step(k)
+                sim.sync()
+                t2 = time.perf_counter()
+                sim.download_f64()
+            t3 = time.perf_counter()
+            step = (t2 - t1) / k
+            rows.append((n, step, (t1 - t0) - 2 * step + (t3 - t2)))
+        pts[solver] = rows
+
+    def cost(n, steps):
+        rows = pts[nb.SOLVER_DIRECT if n < 65536 else nb.SOLVER_BH]
+        x = np.log([r[0] for r in rows])
+        step = np.exp(np.interp(np.log(n), x, np.log([r[1] for r in rows])))
+        setup = np.interp(np.log(n), x, [r[2] for r in rows])
+        return steps * step + setup
+
+    return cost, {str(k): [[int(r[0]), r[1] * 1e3, r[2] * 1e3] for r in v] for k, v in pts.items()}
+
+
+def scheduling_bound(jobs, window: float, cost):
+    """Jobs per second finished within `window` by (a) first-in-first-out one
+    job at a time and (b) preemptive shortest-remaining-work-first, the policy
+    that maximises the number of finished jobs, both with every job costing its
+    one-at-a-time device time (no overlap between jobs)."""
+    import heapq
+
+    arr = [j for j in jobs if j["t"] <= window]
+    c = [cost(j["n"], j["steps"]) for j in arr]
+    t, fifo = 0.0, 0
+    for j, cj in zip(arr, c):
+        t = max(t, j["t"]) + cj
+        if t > window:
+            break
+        fifo += 1
+    h, now, i, done = [], 0.0, 0, 0
+    while now < window:
+        while i < len(arr) and arr[i]["t"] <= now:
+            heapq.heappush(h, c[i])
+            i += 1
+        if not h:
+            if i >= len(arr):
+                break
+            now = arr[i]["t"]
+            continue
+        nxt = min(arr[i]["t"] if i < len(arr) else window, window)
+        r = heapq.heappop(h)
+        if now + r <= nxt:
+            now += r
+            done += 1
+        else:
+            heapq.heappush(h, r - (nxt - now))
+            now = nxt
+    return fifo / window, done / window
+
+
 def bench_serve(nb, rate: float, duration: float, limit: float, lookaheads=(16, 4096)):
     jobs = poisson_jobs(5050, rate, duration)
     out = {"workload": f"Poisson stream {rate:g} jobs/s for {duration:g} s (seeded), N log-uniform "
@@ -514,14 +587,16 @@ def bench_serve(nb, rate: float, duration: float, limit: float, lookaheads=(16, 
            "offered_jobs": len(jobs),
            "offered_body_steps": int(sum(j["n"] * j["steps"] for j in jobs if j["t"] <= limit))}
     ic_threads = 8
-    modes = [("one_at_a_time", None)] + [(f"batched_lookahead{la}", la) for la in lookaheads]
-    for mode, la in modes:
+    modes = [("one_at_a_time", None, 0)] + [(f"batched_lookahead{la}", la, nb.SERVE_ARRIVAL)
+                                            for la in lookaheads]
+    modes.append((f"batched_shortest_lookahead{lookaheads[-1]}", lookaheads[-1], nb.SERVE_SHORTEST))
+    for mode, la, policy in modes:
         log(f"serve: {mode}")
         if la is None:
             done, bs, submitted, queued, running = run_one_at_a_time(nb, jobs, limit, ic_threads)
         else:
             done, bs, submitted, queued, running = run_stream(nb, jobs, limit, lookahead=la,
-                                                              ic_threads=ic_threads)
+                                                              ic_threads=ic_threads, policy=policy)
         lat = np.array([d["latency"] for d in done]) if done else np.array([np.nan])
         out[mode] = {"submitted": submitted, "completed": len(done), "running": running,
                      "queued": queued, "body_steps_per_s": bs / limit,
@@ -530,10 +605,26 @@ def bench_serve(nb, rate: float, duration: float, limit: float, lookaheads=(16, 
                      "latency_p99_s": float(np.percentile(lat, 99))}
         log(f"serve: {mode}: {out[mode]}")
     o = out["one_at_a_time"]
-    for mode, la in modes[1:]:
+    for mode, la, _ in modes[1:]:
         b = out[mode]
         b["ratio_body_steps"] = b["body_steps_per_s"] / max(o["body_steps_per_s"], 1e-9)
         b["ratio_jobs"] = b["completed_per_s"] / max(o["completed_per_s"], 1e-9)
+    log("serve: job costs alone (capacity bound)")
+    cost, table = job_cost_model(nb)
+    fifo, srpt = scheduling_bound(jobs, limit, cost)
+    out["capacity_bound"] = {
+        "what": "jobs/s any scheduler can finish in the window if every job costs its one-at-a-time "
+                "device time: shortest-remaining-work-first (the maximum) and FIFO (the baseline's "
+                "order), from job costs timed alone on this GPU",
+        "srpt_jobs_per_s": srpt, "fifo_jobs_per_s": fifo, "ratio": srpt / max(fifo, 1e-9),
+        "step_ms_and_setup_ms_by_n": table}
+    out["capacity_bound"]["srpt_vs_measured_one_at_a_time"] = srpt / max(o["completed_per_s"], 1e-9)
+    best = max(modes[1:], key=lambda m: out[m[0]]["completed_per_s"])[0]
+    out["best_mode"] = {"mode": best, "ratio_jobs": out[best]["ratio_jobs"],
+                        "fraction_of_srpt_bound": out[best]["completed_per_s"] / max(srpt, 1e-9)}
+    log(f"serve: capacity bound {srpt:.1f} jobs/s (FIFO model {fifo:.1f}, ratio {srpt / max(fifo, 1e-9):.2f}; "
+        f"measured one-at-a-time {o['completed_per_s']:.1f}); best {best}: "
+        f"{out[best]['completed_per_s']:.1f} jobs/s, {out['best_mode']['fraction_of_srpt_bound']:.2f} of the bound")
     return out
 
 

@@ -266,7 +266,16 @@ void nb_batch_destroy(nb_batch* b);
 int nb_batch_add(nb_batch* b, const double* pos3, const double* vel3, const double* acc3,
                  const float* mass, int64_t n, float eps2, float dt, float G, int32_t* id);
 int nb_batch_remove(nb_batch* b, int32_t id);
+/* Advance every cluster nsteps steps, except that a cluster whose quota q is
+ * below nsteps advances q steps (quotas: nb_batch_set_quotas). */
 int nb_batch_step(nb_batch* b, int32_t nsteps);
+/* Quota of steps per nb_batch_step call for the clusters ids[0..count):
+ * steps[k] >= 0, NB_BATCH_UNLIMITED (every cluster's quota when it is added)
+ * for no limit. Quotas persist until changed. A serving loop gives short jobs
+ * many steps per call and long ones one; a cluster's trajectory does not depend
+ * on how its steps are split over calls. */
+#define NB_BATCH_UNLIMITED 0x7fffffff
+int nb_batch_set_quotas(nb_batch* b, const int32_t* ids, const int32_t* steps, int32_t count);
 int nb_batch_sync(nb_batch* b);
 /* Force-only pass for every cluster (state untouched); writes cluster id's
  * forces to force3 (double[3n], interleaved). */
@@ -284,13 +293,15 @@ int nb_batch_get_timing(nb_batch* b, nb_timing* out);
  * admitted in arrival order while their device memory fits the HBM budget and
  * their estimated step time fits the tick budget (a bounded look-ahead lets
  * small jobs pass a large one that does not fit yet), and every admitted job
- * advances one step per nb_server_tick. The tick budget keeps a tick short, so
- * admitted jobs finish in about steps x tick_budget_ms instead of sharing the
- * device with every queued job at once. Direct-sum jobs
- * (n < bh_threshold) share one nb_batch launch; Barnes-Hut jobs run their own
- * contexts on their own streams, so their kernels overlap on the device.
- * Finished jobs release their memory and keep their FP64 final state on the
- * host until fetched. A job's result is the same alone or in any mix.
+ * advances at least one step per nb_server_tick. The tick budget keeps a tick
+ * short, so admitted jobs do not share the device with every queued job at
+ * once. Direct-sum jobs (n < bh_threshold) share one nb_batch launch;
+ * Barnes-Hut jobs run their own contexts on their own streams, so their kernels
+ * overlap on the device. Barnes-Hut contexts are sized in power-of-two
+ * capacity classes and kept in a pool when their job ends, so a steady stream
+ * allocates and frees no device memory. Finished jobs keep their FP64 final
+ * state on the host until fetched. A job's result is the same alone or in any
+ * mix.
  * The reference has no counterpart (one interactive simulation per process,
  * main.cpp:73-153); this is the engine's serving extension. */
 typedef struct nb_server nb_server;
@@ -305,8 +316,22 @@ typedef struct nb_server_config {
   int32_t lookahead;     /* queued jobs considered per admission pass; 0 = 16 */
   float tick_budget_ms;  /* estimated device time of one tick's steps; 0 = 10 ms.
                             A job larger than the budget runs when it is alone. */
-  int32_t pad_;
+  int32_t policy;        /* NB_SERVE_ARRIVAL (0) or NB_SERVE_SHORTEST */
 } nb_server_config;
+
+/* Scheduling policy. ARRIVAL: admission in arrival order among the first
+ * `lookahead` queued jobs (a job that does not fit yet lets later ones pass)
+ * while the admitted jobs' single steps fit the tick budget; every admitted job
+ * advances at least one step per tick, and the budget left over goes to extra
+ * steps of the jobs with the least remaining work (up to 32 per tick), so short
+ * jobs finish in a few ticks. SHORTEST: preemptive shortest remaining work
+ * first: admission in order of estimated total work (steps x step time) up to
+ * four ticks' budget, and each tick's budget goes to the jobs with the least
+ * remaining work, as many steps each as fit; the others wait resident. It
+ * finishes the most jobs per second and can keep a large job waiting while
+ * smaller ones keep arriving. */
+#define NB_SERVE_ARRIVAL 0
+#define NB_SERVE_SHORTEST 1
 
 #define NB_IC_PLUMMER 0 /* a = 50, total mass 1e4, truncated at 10 a */
 #define NB_IC_BALL 1    /* uniform ball, R = 100, m = 1 */
@@ -346,6 +371,15 @@ int nb_server_status(nb_server* s, int64_t id, nb_job_status* out);
 int nb_server_result(nb_server* s, int64_t id, double* pos3, double* vel3);
 int nb_server_forget(nb_server* s, int64_t id);
 double nb_server_now(nb_server* s);
+/* Where the serving loop's time went, summed over its ticks (host wall time,
+ * ms): selection, the idle wait for a first prepared job, starting jobs,
+ * enqueueing steps, waiting for the device, retiring; plus tick and job-step
+ * counts and the idle Barnes-Hut contexts in the pool. */
+typedef struct nb_server_stats {
+  int64_t ticks, steps, pooled_contexts;
+  double select_ms, idle_wait_ms, start_ms, enqueue_ms, device_wait_ms, retire_ms;
+} nb_server_stats;
+int nb_server_get_stats(nb_server* s, nb_server_stats* out);
 
 /* ---- autotuning ---------------------------------------------------------
  * Times candidate launch shapes on the current device with a seeded Plummer

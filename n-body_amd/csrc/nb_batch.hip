@@ -45,13 +45,14 @@ struct nb_batch {
   float* c_eps2 = nullptr;
   double* c_dt = nullptr;
   double* c_G = nullptr;
+  int* c_quota = nullptr;  // steps the cluster takes per nb_batch_step call (INT32_MAX: all)
   // device work tables
   int2* items = nullptr;   // (slot, i-block)
   int2* blocks = nullptr;  // (slot, first body) per 256-body block
-  int* counts = nullptr;   // [0] items, [1] blocks, [2] ticket, [3] flags
+  int* counts = nullptr;   // [0] items, [1] blocks, [2] ticket, [3] flags, [4] step of the call
   double* red = nullptr;   // energy partials
   // host mirror
-  std::vector<int> h_off, h_n, h_npad;
+  std::vector<int> h_off, h_n, h_npad, h_quota;
   std::vector<char> used;
   std::map<int64_t, int64_t> free_list;  // offset -> size
   int64_t used_bodies = 0;
@@ -67,15 +68,21 @@ struct nb_batch {
 
 namespace {
 
+constexpr int kUnlimited = NB_BATCH_UNLIMITED;
+
+// use_quota: skip the clusters whose quota is used up in this call (steps);
+// force-only passes (nb_batch_forces, nb_batch_tune) evaluate every cluster.
 template <int B, int K>
 __global__ __launch_bounds__(64 * K) void k_batch_force(
     const float4* __restrict__ pm, const int* __restrict__ c_off, const int* __restrict__ c_n,
     const int* __restrict__ c_npad, const float* __restrict__ c_eps2,
-    const double* __restrict__ c_G, const int2* __restrict__ items, int* __restrict__ counts,
-    int64_t cap, double* __restrict__ force) {
+    const double* __restrict__ c_G, const int* __restrict__ c_quota, int use_quota,
+    const int2* __restrict__ items, int* __restrict__ counts, int64_t cap,
+    double* __restrict__ force) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int nitems = counts[0];
+  const int call_step = use_quota ? counts[4] : -1;
   __shared__ int s_item;
   __shared__ float part[K][6][B][64];
   for (;;) {
@@ -86,6 +93,7 @@ __global__ __launch_bounds__(64 * K) void k_batch_force(
     if (item >= nitems) break;
     const int2 it = items[item];
     const int slot = __builtin_amdgcn_readfirstlane(it.x);
+    if (__builtin_amdgcn_readfirstlane(c_quota[slot]) <= call_step) continue;  // block-uniform
     const int off = __builtin_amdgcn_readfirstlane(c_off[slot]);
     const int npad = __builtin_amdgcn_readfirstlane(c_npad[slot]);
     const int n = c_n[slot];
@@ -120,39 +128,50 @@ __global__ void k_batch_reset(int* __restrict__ counts) {
   if (threadIdx.x == 0 && blockIdx.x == 0) counts[2] = 0;
 }
 
+// End of one step: the next step of the same nb_batch_step call.
+__global__ void k_batch_advance(int* __restrict__ counts) {
+  if (threadIdx.x == 0 && blockIdx.x == 0 && counts[4] < 0x7fffffff) counts[4] += 1;
+}
+
 template <bool kEuler>
 __global__ __launch_bounds__(256) void k_batch_integrate(
     const int2* __restrict__ blocks, const int* __restrict__ counts, const int* __restrict__ c_off,
-    const int* __restrict__ c_n, const double* __restrict__ c_dt, int64_t cap,
-    double* __restrict__ pos, double* __restrict__ vel, double* __restrict__ acc,
+    const int* __restrict__ c_n, const double* __restrict__ c_dt, const int* __restrict__ c_quota,
+    int64_t cap, double* __restrict__ pos, double* __restrict__ vel, double* __restrict__ acc,
     const double* __restrict__ force, const float* __restrict__ mass, float4* __restrict__ pm) {
-  if ((int)blockIdx.x >= counts[1]) return;
-  const int2 bl = blocks[blockIdx.x];
-  const int i = bl.y + threadIdx.x;
-  if (i - c_off[bl.x] >= c_n[bl.x]) return;  // padding keeps zero mass
-  const double dt = c_dt[bl.x];
-  const double h = dt * 0.5;
-  double x[3];
+  // grid-stride over the pool blocks in use: the launch is sized for the
+  // device, not for the pool's capacity, so an almost empty pool costs little
+  const int nblocks = counts[1];
+  const int step = counts[4];
+  for (int blk = blockIdx.x; blk < nblocks; blk += gridDim.x) {
+    const int2 bl = blocks[blk];
+    if (c_quota[bl.x] <= step) continue;  // quota used up in this call
+    const int i = bl.y + threadIdx.x;
+    if (i - c_off[bl.x] >= c_n[bl.x]) continue;  // padding keeps zero mass
+    const double dt = c_dt[bl.x];
+    const double h = dt * 0.5;
+    double x[3];
 #pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const int64_t k = a * cap + i;
-    double v = vel[k];
-    double p = pos[k];
-    const double f = force[k];
-    if (kEuler) {
-      v = v + f * dt;
-      p = p + v * dt;
-    } else {
-      v = v + acc[k] * h;
-      p = p + v * dt;
-      v = v + f * h;
+    for (int a = 0; a < 3; ++a) {
+      const int64_t k = a * cap + i;
+      double v = vel[k];
+      double p = pos[k];
+      const double f = force[k];
+      if (kEuler) {
+        v = v + f * dt;
+        p = p + v * dt;
+      } else {
+        v = v + acc[k] * h;
+        p = p + v * dt;
+        v = v + f * h;
+      }
+      acc[k] = f;
+      vel[k] = v;
+      pos[k] = p;
+      x[a] = p;
     }
-    acc[k] = f;
-    vel[k] = v;
-    pos[k] = p;
-    x[a] = p;
+    pm[i] = make_float4((float)x[0], (float)x[1], (float)x[2], mass[i]);
   }
-  pm[i] = make_float4((float)x[0], (float)x[1], (float)x[2], mass[i]);
 }
 
 __global__ __launch_bounds__(256) void k_batch_refresh(int off, int n, int npad, int64_t cap,
@@ -200,53 +219,54 @@ __global__ __launch_bounds__(256) void k_batch_energy(int off, int n, int64_t ca
 }
 
 template <int B, int K>
-hipError_t launch_force(nb_batch* b) {
+hipError_t launch_force(nb_batch* b, int use_quota) {
   const int grid = b->cus * 16 / K;  // 16 waves per CU in flight
   hipLaunchKernelGGL((k_batch_force<B, K>), dim3(grid), dim3(64 * K), 0, b->stream, b->pm,
-                     b->c_off, b->c_n, b->c_npad, b->c_eps2, b->c_G, b->items, b->counts, b->cap,
-                     b->force);
+                     b->c_off, b->c_n, b->c_npad, b->c_eps2, b->c_G, b->c_quota, use_quota,
+                     b->items, b->counts, b->cap, b->force);
   return hipGetLastError();
 }
 
 template <int B>
-hipError_t launch_force_b(nb_batch* b) {
+hipError_t launch_force_b(nb_batch* b, int use_quota) {
   switch (b->jsplit) {
-    case 1: return launch_force<B, 1>(b);
-    case 2: return launch_force<B, 2>(b);
-    case 4: return launch_force<B, 4>(b);
+    case 1: return launch_force<B, 1>(b, use_quota);
+    case 2: return launch_force<B, 2>(b, use_quota);
+    case 4: return launch_force<B, 4>(b, use_quota);
     case 8:
-      if constexpr (B <= 2) return launch_force<B, 8>(b);
+      if constexpr (B <= 2) return launch_force<B, 8>(b, use_quota);
       return hipErrorInvalidValue;
     case 16:
-      if constexpr (B == 1) return launch_force<B, 16>(b);
+      if constexpr (B == 1) return launch_force<B, 16>(b, use_quota);
       return hipErrorInvalidValue;
     default: return hipErrorInvalidValue;
   }
 }
 
-hipError_t enqueue_force(nb_batch* b) {
+hipError_t enqueue_force(nb_batch* b, int use_quota = 0) {
   hipLaunchKernelGGL(k_batch_reset, dim3(1), dim3(64), 0, b->stream, b->counts);
   switch (b->bpl) {
-    case 1: return launch_force_b<1>(b);
-    case 2: return launch_force_b<2>(b);
-    case 4: return launch_force_b<4>(b);
+    case 1: return launch_force_b<1>(b, use_quota);
+    case 2: return launch_force_b<2>(b, use_quota);
+    case 4: return launch_force_b<4>(b, use_quota);
     default: return hipErrorInvalidValue;
   }
 }
 
 hipError_t enqueue_step(nb_batch* b) {
-  hipError_t e = enqueue_force(b);
+  hipError_t e = enqueue_force(b, 1);
   if (e != hipSuccess) return e;
   if (b->timing) (void)hipEventRecord(b->ev[1], b->stream);
-  const unsigned nblk = (unsigned)(b->cap / 256);
+  const unsigned nblk = (unsigned)std::min<int64_t>(b->cap / 256, (int64_t)b->cus * 64);
   if (b->cfg.integrator == NB_INTEG_EULER)
     hipLaunchKernelGGL((k_batch_integrate<true>), dim3(nblk), dim3(256), 0, b->stream, b->blocks,
-                       b->counts, b->c_off, b->c_n, b->c_dt, b->cap, b->pos, b->vel, b->acc,
-                       b->force, b->mass, b->pm);
+                       b->counts, b->c_off, b->c_n, b->c_dt, b->c_quota, b->cap, b->pos, b->vel,
+                       b->acc, b->force, b->mass, b->pm);
   else
     hipLaunchKernelGGL((k_batch_integrate<false>), dim3(nblk), dim3(256), 0, b->stream, b->blocks,
-                       b->counts, b->c_off, b->c_n, b->c_dt, b->cap, b->pos, b->vel, b->acc,
-                       b->force, b->mass, b->pm);
+                       b->counts, b->c_off, b->c_n, b->c_dt, b->c_quota, b->cap, b->pos, b->vel,
+                       b->acc, b->force, b->mass, b->pm);
+  hipLaunchKernelGGL(k_batch_advance, dim3(1), dim3(64), 0, b->stream, b->counts);
   return hipGetLastError();
 }
 
@@ -290,7 +310,7 @@ void free_batch(nb_batch* b) {
     if (e) (void)hipEventDestroy(e);
   void* ps[] = {b->pm,     b->pos,    b->vel,   b->acc,   b->force, b->mass,  b->c_off,
                 b->c_n,    b->c_npad, b->c_eps2, b->c_dt, b->c_G,   b->items, b->blocks,
-                b->counts, b->red};
+                b->counts, b->red,    b->c_quota};
   for (void* p : ps)
     if (p) (void)hipFree(p);
   if (b->stream) (void)hipStreamDestroy(b->stream);
@@ -359,6 +379,7 @@ int nb_batch_create(nb_batch** out, const nb_batch_config* cfg) {
       A((void**)&b->c_off, mc * sizeof(int)) || A((void**)&b->c_n, mc * sizeof(int)) ||
       A((void**)&b->c_npad, mc * sizeof(int)) || A((void**)&b->c_eps2, mc * sizeof(float)) ||
       A((void**)&b->c_dt, mc * sizeof(double)) || A((void**)&b->c_G, mc * sizeof(double)) ||
+      A((void**)&b->c_quota, mc * sizeof(int)) ||
       A((void**)&b->items, (cap / 64) * sizeof(int2)) ||
       A((void**)&b->blocks, (cap / 256) * sizeof(int2)) || A((void**)&b->counts, 16 * sizeof(int)) ||
       A((void**)&b->red, (2 * (cap / 256) + 16) * sizeof(double)))
@@ -374,6 +395,7 @@ int nb_batch_create(nb_batch** out, const nb_batch_config* cfg) {
   b->h_n.assign(mc, 0);
   b->h_npad.assign(mc, 0);
   b->used.assign(mc, 0);
+  b->h_quota.assign(mc, kUnlimited);
   b->free_list[0] = b->cap;
   *out = b;
   return NB_OK;
@@ -437,6 +459,7 @@ int nb_batch_add(nb_batch* b, const double* pos3, const double* vel3, const doub
     put(b->c_eps2 + slot, &eps2, sizeof(float));
     put(b->c_dt + slot, &ddt, sizeof(double));
     put(b->c_G + slot, &dG, sizeof(double));
+    put(b->c_quota + slot, &kUnlimited, sizeof(int));
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_batch_refresh, dim3((unsigned)(npad / 256)), dim3(256), 0, b->stream, io,
                        in, inp, b->cap, b->pos, b->mass, b->pm);
@@ -455,6 +478,7 @@ int nb_batch_add(nb_batch* b, const double* pos3, const double* vel3, const doub
   b->h_off[slot] = io;
   b->h_n[slot] = in;
   b->h_npad[slot] = inp;
+  b->h_quota[slot] = kUnlimited;
   b->used_bodies += npad;
   b->dirty = true;
   *id = slot;
@@ -495,6 +519,25 @@ int nb_batch_remove(nb_batch* b, int32_t id) {
   NB_CATCH("nb_batch_remove")
 }
 
+int nb_batch_set_quotas(nb_batch* b, const int32_t* ids, const int32_t* steps, int32_t count) {
+  NB_TRY
+  NB_API_SHARED;
+  if (!b || count < 0 || (count > 0 && (!ids || !steps)))
+    return nb_fail(NB_E_ARG, "nb_batch_set_quotas: bad argument");
+  for (int32_t k = 0; k < count; ++k) {
+    int rc = check_slot(b, ids[k]);
+    if (rc) return rc;
+    if (steps[k] < 0) return nb_fail(NB_E_ARG, "nb_batch_set_quotas: negative quota");
+  }
+  // the previous upload of the table may still read the host mirror
+  NB_HIP(hipStreamSynchronize(b->stream));
+  for (int32_t k = 0; k < count; ++k) b->h_quota[ids[k]] = steps[k];
+  NB_HIP(hipMemcpyAsync(b->c_quota, b->h_quota.data(), b->h_quota.size() * sizeof(int),
+                        hipMemcpyHostToDevice, b->stream));
+  return NB_OK;
+  NB_CATCH("nb_batch_set_quotas")
+}
+
 int nb_batch_step(nb_batch* b, int32_t nsteps) {
   NB_TRY
   NB_RANGE("nb_batch_step");
@@ -506,6 +549,8 @@ int nb_batch_step(nb_batch* b, int32_t nsteps) {
     if (rc) return rc;
   }
   if (nsteps == 0) return NB_OK;
+  // the steps of this call count from 0 for the clusters' quotas
+  NB_HIP(nb_launch_fill_i32(b->counts + 4, 0, 1, b->stream));
   if (b->timing) {
     NB_API_SHARED;
     for (int s = 0; s < nsteps; ++s) {

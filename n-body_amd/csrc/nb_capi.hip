@@ -161,6 +161,65 @@ static int accumulate_timing(nb_ctx* c) {
   return NB_OK;
 }
 
+int nb_prepare_step_graph(nb_ctx* c) {
+  if (c->graph_exec || (c->cfg.flags & NB_F_NO_GRAPH)) return NB_OK;
+  NB_API_CAPTURE;
+  NB_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = nb_enqueue_step(c);
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return nb_fail(NB_E_HIP, std::string("nb_step: graph capture failed: ") +
+                                 hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  const hipError_t ei = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+  if (ei != hipSuccess) {
+    (void)hipGraphDestroy(g);
+    c->graph_exec = nullptr;
+    return nb_fail(NB_E_HIP, std::string("nb_step: graph instantiation failed: ") +
+                                 hipGetErrorString(ei));
+  }
+  c->graph = g;
+  return NB_OK;
+}
+
+int nb_ctx_rebind(nb_ctx* c, const nb_config* cfg) {
+  NB_TRY
+  NB_API_SHARED;
+  if (!c || !cfg || c->cfg.solver != NB_SOLVER_BH || cfg->solver != NB_SOLVER_BH)
+    return nb_fail(NB_E_ARG, "nb_ctx_rebind: Barnes-Hut contexts only");
+  if (cfg->n < 1 || cfg->n > c->cap_n) return nb_fail(NB_E_ARG, "nb_ctx_rebind: n exceeds the capacity");
+  nb_drop_graph(c->stream, c->graph_exec, c->graph);
+  NB_HIP(hipStreamSynchronize(c->stream));
+  c->cfg = *cfg;
+  c->n = (int)cfg->n;
+  c->n_pad = (int)((cfg->n + kPadBodies - 1) / kPadBodies * kPadBodies);
+  c->G = cfg->G == 0.f ? 1.0 : (double)cfg->G;
+  c->cfg.theta = cfg->theta == 0.f ? 0.5f : cfg->theta;
+  c->theta2 = c->cfg.theta * c->cfg.theta;
+  c->guard = !(cfg->eps2 >= 1e-20f);
+  if (c->cfg.leaf_size == 0) {
+    int bpl = 0, js = 0, leaf = 0;
+    nb_tuned_shape(c->cfg.n, &bpl, &js, &leaf);
+    nb_default_shape(c->cfg.n, &bpl, &js, &leaf);
+    c->cfg.leaf_size = leaf;
+  }
+  void* block = c->bh.block;
+  c->bh = nb_bh_state{};
+  c->bh.block = block;
+  nb_bh_carve(c, static_cast<char*>(block), kBhBufCount);
+  NB_HIP(nb_launch_fill_i32(reinterpret_cast<int*>(c->bh.pms + c->n), 0, 4 * kSpare, c->stream));
+  NB_HIP(nb_launch_fill_i32(c->flags, 0, 4, c->stream));
+  NB_HIP(hipStreamSynchronize(c->stream));
+  c->uploaded = false;
+  c->tree_valid = false;
+  c->order.clear();
+  c->tim = nb_timing{};
+  return NB_OK;
+  NB_CATCH("nb_ctx_rebind")
+}
+
 extern "C" {
 
 int nb_create(nb_ctx** out, const nb_config* cfg) {
@@ -206,6 +265,7 @@ int nb_create(nb_ctx** out, const nb_config* cfg) {
   nb_ctx* c = new (std::nothrow) nb_ctx();
   if (!c) return nb_fail(NB_E_OOM, "nb_create: host allocation failed");
   c->cfg = *cfg;
+  c->cap_n = cfg->n;
   c->n = (int)cfg->n;
   c->n_pad = (int)((cfg->n + kPadBodies - 1) / kPadBodies * kPadBodies);
   c->G = cfg->G == 0.f ? 1.0 : (double)cfg->G;
@@ -337,26 +397,8 @@ int nb_step(nb_ctx* c, int32_t nsteps) {
   if (!use_graph) {
     for (int s = 0; s < nsteps; ++s) NB_HIP(nb_enqueue_step(c));
   } else {
-    if (!c->graph_exec) {
-      NB_API_CAPTURE;
-      NB_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      hipError_t e = nb_enqueue_step(c);
-      hipGraph_t g = nullptr;
-      hipError_t e2 = hipStreamEndCapture(c->stream, &g);
-      if (e != hipSuccess || e2 != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        return nb_fail(NB_E_HIP, std::string("nb_step: graph capture failed: ") +
-                                     hipGetErrorString(e != hipSuccess ? e : e2));
-      }
-      const hipError_t ei = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
-      if (ei != hipSuccess) {
-        (void)hipGraphDestroy(g);
-        c->graph_exec = nullptr;
-        return nb_fail(NB_E_HIP, std::string("nb_step: graph instantiation failed: ") +
-                                     hipGetErrorString(ei));
-      }
-      c->graph = g;
-    }
+    const int rc = nb_prepare_step_graph(c);
+    if (rc) return rc;
     for (int s = 0; s < nsteps; ++s) NB_HIP(hipGraphLaunch(c->graph_exec, c->stream));
   }
   c->tree_valid = c->cfg.solver == NB_SOLVER_BH;

@@ -181,6 +181,7 @@ struct nb_bh_state {
 
 struct nb_ctx {
   nb_config cfg{};
+  int64_t cap_n = 0;  // bodies the device buffers were sized for (nb_ctx_rebind limit)
   int n = 0;
   int n_pad = 0;
   int device = 0;
@@ -268,3 +269,9 @@ bool nb_valid_shape(int bpl, int jsplit);  // a direct-sum kernel exists for it
 
 // step = forces + integration (no sync)
 hipError_t nb_enqueue_step(nb_ctx* c);
+// Capture and instantiate the one-step graph if there is none (launches nothing).
+int nb_prepare_step_graph(nb_ctx* c);
+// Reuse a Barnes-Hut context for another system of the same solver:
+// cfg->n <= c->cap_n; the buffers are re-laid out for cfg->n, the captured step
+// graph is dropped and the state must be uploaded again. Allocates nothing.
+int nb_ctx_rebind(nb_ctx* c, const nb_config* cfg);

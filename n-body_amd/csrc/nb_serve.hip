@@ -13,16 +13,22 @@
 //   2. start   selected jobs whose initial conditions are ready start running:
 //              direct-sum jobs become clusters of the shared nb_batch pool;
 //              Barnes-Hut jobs have their own nb_ctx (own stream), which the
-//              worker that made their initial conditions already created and
-//              uploaded;
-//   3. step    one nb_batch_step for every direct-sum job at once (one persistent
-//              force launch) and one nb_step per Barnes-Hut job, all enqueued
-//              before any wait, so Barnes-Hut kernels overlap with each other and
-//              with the batch;
+//              worker that made their initial conditions already took from the
+//              context pool (or created), uploaded, and captured the step of;
+//   3. step    every running job gets one step, and the tick budget left over
+//              goes to extra steps for the jobs with the least remaining work
+//              (at most kMaxQuota per tick); direct-sum jobs step together in
+//              one nb_batch_step (per-cluster quotas), Barnes-Hut jobs in one
+//              nb_step each, all enqueued before any wait, so Barnes-Hut kernels
+//              overlap with each other and with the batch;
 //   4. retire  jobs that reached their step count copy their FP64 state to the
-//              host and release their reservations; a Barnes-Hut job's copy and
-//              context teardown run on a worker thread, so the loop never waits
-//              on a large transfer while the device could be stepping.
+//              host and release their reservations; a Barnes-Hut job's copy runs
+//              on a worker thread, so the loop never waits on a large transfer,
+//              and its context goes back to the pool.
+// Barnes-Hut contexts are sized in power-of-two capacity classes and rebound
+// to each new job of their class (nb_ctx_rebind): a steady stream allocates
+// and frees no device memory; idle pooled contexts are freed only when a new
+// class does not fit the HBM budget.
 // Only selected jobs hold host memory for initial conditions, so a long queue
 // costs a few bytes per job.
 #include <algorithm>
@@ -47,13 +53,17 @@ struct Job {
   int32_t state = NB_JOB_QUEUED;  // preparing jobs report NB_JOB_QUEUED
   int32_t solver = NB_SOLVER_DIRECT;
   int32_t steps_done = 0;
+  int32_t quota = 1;        // steps in the current tick
   double t_submit = 0, t_admit = 0, t_done = 0;
   // reservations held from selection to retirement
   int64_t bytes = 0;        // HBM budget (Barnes-Hut contexts)
   int64_t pool_bodies = 0;  // direct-sum pool bodies (padded)
   double cost_ms = 0;       // estimated device time of one step
   int32_t slot = -1;        // batch cluster id
-  nb_ctx* ctx = nullptr;
+  nb_ctx* ctx = nullptr;    // Barnes-Hut context (from the pool when `reused`)
+  int64_t cap = 0;          // its capacity class
+  bool reused = false;
+  bool in_pool = false;     // its context went back to the pool (keeps its bytes)
   // initial conditions (made by a worker) and, later, the final state
   std::atomic<int> prepared{0};  // 0 pending, 1 ready, -1 failed
   std::atomic<int> finished{0};  // Barnes-Hut retirement on a worker: 1 done, -1 failed
@@ -62,19 +72,21 @@ struct Job {
   std::vector<float> mass;
 };
 
-int64_t tree_bound(int64_t n) {
-  int64_t total = 0, w = 1;
-  for (int l = 0; l <= kMaxLevel; ++l) {
-    total += w < n ? w : n;
-    if (w < n) w *= 8;
-  }
-  return total;
+// Capacity class of a Barnes-Hut job: the power of two at or above n (at least
+// 4096), so pooled contexts fit the jobs of their class.
+int64_t capacity(int64_t n) {
+  int64_t c = int64_t(1) << 12;
+  while (c < n) c <<= 1;
+  return c;
 }
 
-// Device bytes of a Barnes-Hut context of n bodies (nb_create + nb_bh_alloc).
+// Device bytes of a Barnes-Hut context sized for n bodies (nb_create +
+// nb_bh_alloc): FP64 state and forces, FP32 view and masses, reductions, and the
+// single octree block.
 int64_t bh_bytes(int64_t n) {
   const int64_t np = (n + 255) / 256 * 256;
-  return np * (16 + 4) + n * (4 * 24 + 4 * 5 + 16 + 32 + 8 + 32) + 32 * tree_bound(n) + (4 << 20);
+  return 4 * 24 * n + np * (16 + 4) + 8 * (4 * 256 + np / 256 + 16) +
+         nb_bh_layout_for(n).off[kBhBufCount] + (1 << 20);
 }
 
 int64_t padded(int64_t n) { return (n + kPadBodies - 1) / kPadBodies * kPadBodies; }
@@ -82,11 +94,17 @@ int64_t padded(int64_t n) { return (n + kPadBodies - 1) / kPadBodies * kPadBodie
 // Estimated device time of one step, from rates measured on MI355X: the
 // compensated direct sum runs about 3.2e12 pair interactions/s once the device
 // is full, and a Barnes-Hut step (theta = 0.5, build + traversal) costs about
-// 2 ns per body plus some 60 us of launches (8.5 ms at 4,194,304 bodies).
+// 1.7 ns per body plus some 0.35 ms of latency-bound launches (7.3 ms at
+// 4,194,304 bodies, about 0.5 ms at 65,536).
 double step_cost_ms(int64_t n, int32_t solver) {
   if (solver == NB_SOLVER_DIRECT) return 2e-3 + (double)n * (double)n / 3.2e9;
-  return 0.06 + (double)n * 2.0e-6;
+  return 0.35 + (double)n * 1.7e-6;
 }
+
+// Most steps a job takes in one tick (bounds the batch's replays per tick).
+constexpr int kMaxQuota = 32;
+// Shortest-first admits jobs up to this many ticks' budget of single steps.
+constexpr double kOversubscribe = 4.0;
 
 }  // namespace
 
@@ -113,13 +131,19 @@ struct nb_server {
   std::deque<Job*> retire_queue;
   bool stop = false;
   std::vector<std::thread> workers;
+  // idle Barnes-Hut contexts by capacity class; workers return them (under mu),
+  // the loop takes and frees them. Their bytes stay out of bytes_left.
+  std::multimap<int64_t, nb_ctx*> ctx_pool;
+  nb_server_stats stats{};
 
   double now() const {
     return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   }
 
-  // Barnes-Hut context for a prepared job: created and uploaded here, on the
-  // job's own stream, so the serving loop does not wait for the copy.
+  // Barnes-Hut context for a prepared job: the pooled context the loop handed
+  // over, rebound to this job, or a new one of the job's capacity class; then
+  // the upload and the step-graph capture, all on the job's own stream, so the
+  // serving loop does not wait for them.
   int make_context(Job& j) {
     nb_config c{};
     c.n = j.desc.n;
@@ -129,15 +153,20 @@ struct nb_server {
     c.theta = cfg.theta;
     c.leaf_size = cfg.leaf_size;
     c.solver = NB_SOLVER_BH;
-    nb_ctx* ctx = nullptr;
-    int rc = nb_create(&ctx, &c);
-    if (rc != NB_OK) return rc;
-    rc = nb_upload_f64(ctx, j.pos.data(), j.vel.data(), nullptr, j.mass.data(), j.desc.n);
-    if (rc != NB_OK) {
-      nb_destroy(ctx);
-      return rc;
+    nb_ctx* ctx = j.ctx;
+    int rc = NB_OK;
+    if (!ctx) {
+      nb_config cc = c;
+      cc.n = j.cap;
+      rc = nb_create(&ctx, &cc);
+      if (rc != NB_OK) return rc;
+      j.ctx = ctx;
     }
-    j.ctx = ctx;
+    rc = nb_ctx_rebind(ctx, &c);
+    if (rc == NB_OK)
+      rc = nb_upload_f64(ctx, j.pos.data(), j.vel.data(), nullptr, j.mass.data(), j.desc.n);
+    if (rc == NB_OK) rc = nb_prepare_step_graph(ctx);
+    if (rc != NB_OK) return rc;  // the loop destroys j.ctx when it retires the failed job
     std::vector<double>().swap(j.pos);
     std::vector<double>().swap(j.vel);
     return NB_OK;
@@ -166,11 +195,14 @@ struct nb_server {
         } catch (const std::bad_alloc&) {
           rc = NB_E_OOM;
         }
-        nb_destroy(j->ctx);
-        j->ctx = nullptr;
         j->t_fin = now();
         {
           std::lock_guard<std::mutex> lk(mu);
+          if (rc == NB_OK) {  // the context serves the next job of its class
+            ctx_pool.emplace(j->cap, j->ctx);
+            j->ctx = nullptr;
+            j->in_pool = true;
+          }
           j->finished.store(rc == NB_OK ? 1 : -1);
         }
         ready_cv.notify_all();
@@ -206,9 +238,47 @@ struct nb_server {
   }
   void release(Job& j) {
     release_time(j);
-    bytes_left += j.bytes;
+    if (!j.in_pool) bytes_left += j.bytes;  // a pooled context keeps its bytes reserved
     pool_left += j.pool_bodies;
     j.bytes = j.pool_bodies = 0;
+  }
+
+  // Reserve the device memory of a Barnes-Hut job: an idle pooled context of
+  // its class if there is one, else the bytes of a new one, freeing idle pooled
+  // contexts of other classes when the budget needs them. False: no room yet.
+  bool reserve_bh(Job& j) {
+    j.cap = capacity(j.desc.n);
+    j.bytes = bh_bytes(j.cap);
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      auto it = ctx_pool.find(j.cap);
+      if (it != ctx_pool.end()) {
+        j.ctx = it->second;
+        j.reused = true;
+        ctx_pool.erase(it);
+        return true;
+      }
+    }
+    while (j.bytes > bytes_left) {
+      nb_ctx* victim = nullptr;
+      int64_t vcap = 0;
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        if (ctx_pool.empty()) break;
+        auto it = std::prev(ctx_pool.end());  // the largest idle context
+        vcap = it->first;
+        victim = it->second;
+        ctx_pool.erase(it);
+      }
+      nb_destroy(victim);
+      bytes_left += bh_bytes(vcap);
+    }
+    if (j.bytes > bytes_left) {
+      j.bytes = 0;
+      return false;
+    }
+    bytes_left -= j.bytes;
+    return true;
   }
 };
 
@@ -244,7 +314,7 @@ int retire(nb_server* s, Job& j, int rc = NB_OK) {
       if (rc == NB_OK) rc = rr;
       j.slot = -1;
     }
-  } else if (j.ctx) {  // a failed Barnes-Hut job: no copy
+  } else if (j.ctx) {  // a failed Barnes-Hut job: no copy, and its context is not reused
     nb_destroy(j.ctx);
     j.ctx = nullptr;
   }
@@ -270,6 +340,7 @@ void free_server(nb_server* s) {
   for (auto& t : s->workers) t.join();
   for (auto& kv : s->jobs)
     if (kv.second->ctx) nb_destroy(kv.second->ctx);
+  for (auto& kv : s->ctx_pool) nb_destroy(kv.second);
   if (s->batch) nb_batch_destroy(s->batch);
   delete s;
 }
@@ -292,6 +363,10 @@ int nb_server_create(nb_server** out, const nb_server_config* cfg) {
   if (s->cfg.ic_threads <= 0) s->cfg.ic_threads = 4;
   if (s->cfg.lookahead <= 0) s->cfg.lookahead = 16;
   if (!(s->cfg.tick_budget_ms > 0.f)) s->cfg.tick_budget_ms = 10.f;
+  if (s->cfg.policy != NB_SERVE_ARRIVAL && s->cfg.policy != NB_SERVE_SHORTEST) {
+    delete s;
+    return nb_fail(NB_E_ARG, "nb_server_create: unknown policy");
+  }
   s->t0 = std::chrono::steady_clock::now();
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
@@ -341,7 +416,7 @@ int nb_server_submit(nb_server* s, const nb_job* job, int64_t* id) {
       (job->ic != NB_IC_PLUMMER && job->ic != NB_IC_BALL) || !(job->eps2 >= 0.f))
     return nb_fail(NB_E_ARG, "nb_server_submit: bad job description");
   const int32_t solver = job->n >= s->cfg.bh_threshold ? NB_SOLVER_BH : NB_SOLVER_DIRECT;
-  if (solver == NB_SOLVER_BH ? bh_bytes(job->n) > s->budget_total
+  if (solver == NB_SOLVER_BH ? bh_bytes(capacity(job->n)) > s->budget_total
                              : padded(job->n) > s->cfg.direct_pool)
     return nb_fail(NB_E_BUDGET, "nb_server_submit: the job can never fit the HBM budget");
   auto j = std::make_unique<Job>();
@@ -365,6 +440,14 @@ int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
   NB_RANGE("nb_server_tick");
   if (!s) return nb_fail(NB_E_ARG, "nb_server_tick: null server");
   int finished = 0;
+  using clk = std::chrono::steady_clock;
+  auto t_mark = clk::now();
+  auto lap = [&](double& acc) {
+    const auto t = clk::now();
+    acc += std::chrono::duration<double, std::milli>(t - t_mark).count();
+    t_mark = t;
+  };
+  s->stats.ticks += 1;
   // 0. Barnes-Hut jobs whose worker has copied them out are done
   auto collect = [&] {
     for (auto it = s->retiring.begin(); it != s->retiring.end();) {
@@ -381,36 +464,54 @@ int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
     }
   };
   collect();
-  // 1. select (arrival order, bounded look-ahead)
-  int scanned = 0;
-  for (auto it = s->queue.begin(); it != s->queue.end() && scanned < s->cfg.lookahead;) {
-    if ((int)(s->running.size() + s->preparing.size()) >= s->cfg.max_jobs) break;
-    ++scanned;
-    Job& j = *s->jobs.at(*it);
-    const bool idle = s->running.empty() && s->preparing.empty();
-    const bool fits_tick = idle || s->load_ms + j.cost_ms <= s->cfg.tick_budget_ms;
-    const bool fits_mem = j.solver == NB_SOLVER_BH ? bh_bytes(j.desc.n) <= s->bytes_left
-                                                   : padded(j.desc.n) <= s->pool_left;
-    if (!fits_tick || !fits_mem) {
-      ++it;  // a later, smaller job may fit
-      continue;
+  // 1. select among the first `lookahead` queued jobs, in arrival order or
+  //    smallest estimated total work first (policy)
+  {
+    std::vector<int64_t> cand;
+    for (auto it = s->queue.begin(); it != s->queue.end() && (int)cand.size() < s->cfg.lookahead; ++it)
+      cand.push_back(*it);
+    if (s->cfg.policy == NB_SERVE_SHORTEST)
+      std::stable_sort(cand.begin(), cand.end(), [&](int64_t a, int64_t b) {
+        const Job& x = *s->jobs.at(a);
+        const Job& y = *s->jobs.at(b);
+        return x.cost_ms * x.desc.steps < y.cost_ms * y.desc.steps;
+      });
+    std::vector<int64_t> taken;
+    for (int64_t id : cand) {
+      if ((int)(s->running.size() + s->preparing.size()) >= s->cfg.max_jobs) break;
+      Job& j = *s->jobs.at(id);
+      const bool idle = s->running.empty() && s->preparing.empty();
+      // shortest-first oversubscribes: admitted jobs beyond one tick's budget
+      // wait resident (no steps) while jobs with less remaining work run
+      const double gate = s->cfg.policy == NB_SERVE_SHORTEST ? kOversubscribe * s->cfg.tick_budget_ms
+                                                             : s->cfg.tick_budget_ms;
+      const bool fits_tick = idle || s->load_ms + j.cost_ms <= gate;
+      if (!fits_tick) continue;  // a later, smaller job may fit
+      if (j.solver == NB_SOLVER_BH) {
+        if (!s->reserve_bh(j)) continue;
+      } else {
+        if (padded(j.desc.n) > s->pool_left) continue;
+        j.pool_bodies = padded(j.desc.n);
+        s->pool_left -= j.pool_bodies;
+      }
+      s->load_ms += j.cost_ms;
+      s->preparing.push_back(id);
+      {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->prep_queue.push_back(&j);
+      }
+      s->cv.notify_one();
+      taken.push_back(id);
     }
-    if (j.solver == NB_SOLVER_BH) {
-      j.bytes = bh_bytes(j.desc.n);
-      s->bytes_left -= j.bytes;
-    } else {
-      j.pool_bodies = padded(j.desc.n);
-      s->pool_left -= j.pool_bodies;
+    if (!taken.empty()) {
+      std::sort(taken.begin(), taken.end());
+      std::deque<int64_t> rest;
+      for (int64_t id : s->queue)
+        if (!std::binary_search(taken.begin(), taken.end(), id)) rest.push_back(id);
+      s->queue.swap(rest);
     }
-    s->load_ms += j.cost_ms;
-    s->preparing.push_back(*it);
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      s->prep_queue.push_back(&j);
-    }
-    s->cv.notify_one();
-    it = s->queue.erase(it);
   }
+  lap(s->stats.select_ms);
   // Idle with work on the workers: wait (up to 50 ms) for the first prepared
   // or copied-out job instead of returning an empty tick.
   if (s->running.empty() && (!s->preparing.empty() || !s->retiring.empty())) {
@@ -426,6 +527,7 @@ int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
     }
     collect();
   }
+  lap(s->stats.idle_wait_ms);
   // 2. start the prepared jobs (selection order)
   for (auto it = s->preparing.begin(); it != s->preparing.end();) {
     Job& j = *s->jobs.at(*it);
@@ -451,37 +553,73 @@ int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
     }
     it = s->preparing.erase(it);
   }
-  // 3. one step for every running job, everything enqueued before any wait
-  bool any_direct = false;
+  lap(s->stats.start_ms);
+  // 3. steps, least remaining work first. ARRIVAL: one for every running job,
+  //    and the tick budget their single steps leave goes to extra steps, so
+  //    short jobs finish in a few ticks. SHORTEST (preemptive shortest remaining
+  //    work first): the tick budget goes to the jobs in that order, as many
+  //    steps each as fit, and a job it does not reach waits this tick (the
+  //    first always steps). Everything is enqueued before any wait.
+  std::vector<Job*> act;
+  double base_ms = 0;
+  const bool srpt = s->cfg.policy == NB_SERVE_SHORTEST;
   for (int64_t id : s->running) {
     Job& j = *s->jobs.at(id);
     if (j.steps_done >= j.desc.steps) continue;
-    if (j.solver == NB_SOLVER_DIRECT) {
-      any_direct = true;
-    } else if (nb_step(j.ctx, 1) != NB_OK) {
-      j.state = NB_JOB_FAILED;  // this job alone; it is retired below
+    j.quota = srpt ? 0 : 1;
+    base_ms += srpt ? 0.0 : j.cost_ms;
+    act.push_back(&j);
+  }
+  std::stable_sort(act.begin(), act.end(), [](const Job* a, const Job* b) {
+    return a->cost_ms * (a->desc.steps - a->steps_done) < b->cost_ms * (b->desc.steps - b->steps_done);
+  });
+  double spare_ms = s->cfg.tick_budget_ms - base_ms;
+  for (Job* j : act) {
+    if (spare_ms <= 0 && !(srpt && j == act.front())) break;
+    const int left = j->desc.steps - j->steps_done;
+    const int have = j->quota;
+    int add = (int)std::min<double>({(double)(left - have), (double)(kMaxQuota - have),
+                                     std::floor(spare_ms / j->cost_ms)});
+    if (srpt && j == act.front()) add = std::max(add, 1);
+    if (add <= 0) continue;
+    j->quota += add;
+    spare_ms -= add * j->cost_ms;
+  }
+  std::vector<int32_t> qid, qsteps;
+  int batch_steps = 0;
+  for (Job* j : act) {
+    if (j->solver == NB_SOLVER_DIRECT) {
+      qid.push_back(j->slot);
+      qsteps.push_back(j->quota);
+      batch_steps = std::max(batch_steps, j->quota);
+    } else if (j->quota > 0 && nb_step(j->ctx, j->quota) != NB_OK) {
+      j->state = NB_JOB_FAILED;  // this job alone; it is retired below
     }
   }
-  if (any_direct) {
-    const int rc = nb_batch_step(s->batch, 1);
+  if (batch_steps > 0) {
+    int rc = nb_batch_set_quotas(s->batch, qid.data(), qsteps.data(), (int32_t)qid.size());
+    if (rc == NB_OK) rc = nb_batch_step(s->batch, batch_steps);
     if (rc != NB_OK) return rc;
   }
-  for (int64_t id : s->running) {
-    Job& j = *s->jobs.at(id);
-    if (j.steps_done >= j.desc.steps || j.state == NB_JOB_FAILED) continue;
-    if (j.solver == NB_SOLVER_BH) {
-      const int rc = nb_sync(j.ctx);
+  lap(s->stats.enqueue_ms);
+  for (Job* j : act) {
+    s->stats.steps += j->quota;
+    if (j->state == NB_JOB_FAILED || j->quota == 0) continue;
+    if (j->solver == NB_SOLVER_BH) {
+      const int rc = nb_sync(j->ctx);
       if (rc != NB_OK) {
-        j.state = NB_JOB_FAILED;
+        j->state = NB_JOB_FAILED;
         continue;
       }
     }
-    ++j.steps_done;
+    j->steps_done += j->quota;
   }
+  const bool any_direct = batch_steps > 0;
   if (any_direct) {
     const int rc = nb_batch_sync(s->batch);
     if (rc != NB_OK) return rc;
   }
+  lap(s->stats.device_wait_ms);
   // 4. retire: direct-sum and failed jobs here, Barnes-Hut jobs on a worker
   for (auto it = s->running.begin(); it != s->running.end();) {
     Job& j = *s->jobs.at(*it);
@@ -504,10 +642,23 @@ int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
       ++it;
     }
   }
+  lap(s->stats.retire_ms);
   if (running_out) *running_out = (int32_t)s->running.size();
   if (finished_out) *finished_out = finished;
   return NB_OK;
   NB_CATCH("nb_server_tick")
+}
+
+int nb_server_get_stats(nb_server* s, nb_server_stats* out) {
+  NB_TRY
+  if (!s || !out) return nb_fail(NB_E_ARG, "nb_server_get_stats: null argument");
+  *out = s->stats;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    out->pooled_contexts = (int64_t)s->ctx_pool.size();
+  }
+  return NB_OK;
+  NB_CATCH("nb_server_get_stats")
 }
 
 int nb_server_poll(nb_server* s, int64_t* ids, int32_t cap, int32_t* count) {

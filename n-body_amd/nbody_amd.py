@@ -37,6 +37,8 @@ F_BH_PER_BODY = 0x4
 F_BH_AXIS_BOX = 0x8
 BH_WAVE_BODIES = 32  # NB_BH_WAVE_BODIES
 NOALLOC_MAX_BODIES = 65536  # NB_NOALLOC_MAX_BODIES
+BATCH_UNLIMITED = 0x7FFFFFFF  # NB_BATCH_UNLIMITED
+SERVE_ARRIVAL, SERVE_SHORTEST = 0, 1  # NB_SERVE_*
 
 
 class NBodyError(RuntimeError):
@@ -124,7 +126,21 @@ class ServerConfig(C.Structure):
         ("ic_threads", C.c_int32),
         ("lookahead", C.c_int32),
         ("tick_budget_ms", C.c_float),
-        ("pad_", C.c_int32),
+        ("policy", C.c_int32),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("ticks", C.c_int64),
+        ("steps", C.c_int64),
+        ("pooled_contexts", C.c_int64),
+        ("select_ms", C.c_double),
+        ("idle_wait_ms", C.c_double),
+        ("start_ms", C.c_double),
+        ("enqueue_ms", C.c_double),
+        ("device_wait_ms", C.c_double),
+        ("retire_ms", C.c_double),
     ]
 
 
@@ -210,6 +226,7 @@ _SIGS = {
                                C.POINTER(C.c_int32)]),
     "nb_batch_remove": (C.c_int, [_P, C.c_int32]),
     "nb_batch_step": (C.c_int, [_P, C.c_int32]),
+    "nb_batch_set_quotas": (C.c_int, [_P, _P, _P, C.c_int32]),
     "nb_batch_sync": (C.c_int, [_P]),
     "nb_batch_forces": (C.c_int, [_P, C.c_int32, _P]),
     "nb_batch_download": (C.c_int, [_P, C.c_int32, _P, _P, _P]),
@@ -226,6 +243,7 @@ _SIGS = {
     "nb_server_status": (C.c_int, [_P, C.c_int64, C.POINTER(JobStatus)]),
     "nb_server_result": (C.c_int, [_P, C.c_int64, _P, _P]),
     "nb_server_forget": (C.c_int, [_P, C.c_int64]),
+    "nb_server_get_stats": (C.c_int, [_P, C.POINTER(ServerStats)]),
     "nb_server_now": (C.c_double, [_P]),
     "nb_autotune": (C.c_int, [C.c_int64, C.c_int32, C.c_int32, C.POINTER(Tuning)]),
     "nb_get_tuning": (C.c_int, [C.c_int64, C.POINTER(Tuning)]),
@@ -625,6 +643,13 @@ class Batch:
     def step(self, nsteps: int = 1):
         _check(self._lib.nb_batch_step(self._b, nsteps))
 
+    def set_quotas(self, quotas: dict):
+        """Steps per step() call for the given clusters ({id: steps};
+        BATCH_UNLIMITED lifts the limit)."""
+        ids = np.ascontiguousarray(list(quotas.keys()), np.int32)
+        st = np.ascontiguousarray(list(quotas.values()), np.int32)
+        _check(self._lib.nb_batch_set_quotas(self._b, _ptr(ids), _ptr(st), len(ids)))
+
     def tune(self, reps: int = 3) -> dict:
         """Time every launch shape on the current clusters and keep the fastest."""
         t = Tuning()
@@ -672,9 +697,9 @@ class Server:
 
     def __init__(self, hbm_budget: int = 0, direct_pool: int = 0, max_jobs: int = 0,
                  bh_threshold: int = 0, theta: float = 0.0, leaf_size: int = 0, ic_threads: int = 0,
-                 lookahead: int = 0, tick_budget_ms: float = 0.0):
+                 lookahead: int = 0, tick_budget_ms: float = 0.0, policy: int = 0):
         self._lib = load()
-        cfg = ServerConfig(hbm_budget=hbm_budget, direct_pool=direct_pool, max_jobs=max_jobs,
+        cfg = ServerConfig(hbm_budget=hbm_budget, direct_pool=direct_pool, max_jobs=max_jobs, policy=policy,
                            bh_threshold=bh_threshold, theta=theta, leaf_size=leaf_size,
                            ic_threads=ic_threads, lookahead=lookahead,
                            tick_budget_ms=tick_budget_ms)
@@ -739,3 +764,9 @@ class Server:
 
     def now(self) -> float:
         return self._lib.nb_server_now(self._s)
+
+    def stats(self) -> dict:
+        """Per-phase host time of the serving loop (nb_server_get_stats)."""
+        st = ServerStats()
+        _check(self._lib.nb_server_get_stats(self._s, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}

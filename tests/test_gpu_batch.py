@@ -123,3 +123,70 @@ def test_tune_picks_a_shape_and_keeps_results(nb, ora):
                 f = ora.direct_f64(op.astype(np.float32).astype(np.float64), m.astype(np.float64), e)
                 ora.kdk_update_f64(op, ov, oa, f, d)
             assert np.abs(gp - op).max() < 1e-9 * max(1.0, np.abs(op).max())
+
+
+def test_quotas_split_steps_without_changing_trajectories(nb):
+    """Clusters with step quotas advance min(quota, nsteps) steps per call; after
+    three calls with changing quotas each cluster equals itself stepped the same
+    number of times alone, bit for bit. A quota of 0 freezes a cluster."""
+    cl = make_clusters(nb, 6, 256, 2000, 13)
+    quotas = [[1, 4, 0, 8, nb.BATCH_UNLIMITED, 3], [2, 4, 5, 8, 1, 0], [8, 1, 1, 8, 8, 8]]
+    with nb.Batch(sum(len(c[2]) + 256 for c in cl), 8) as b:
+        ids = [b.add(p, v, m, e, d) for p, v, m, e, d in cl]
+        total = [0] * len(ids)
+        for q in quotas:
+            b.set_quotas(dict(zip(ids, q)))
+            b.step(6)
+            for k in range(len(ids)):
+                total[k] += min(q[k], 6)
+        b.sync()
+        mixed = [b.download(cid) for cid in ids]
+    assert total == [9, 9, 6, 18, 13, 9]
+    for k, (p, v, m, e, d) in enumerate(cl):
+        with nb.Batch(len(m) + 256, 1) as b:
+            cid = b.add(p, v, m, e, d)
+            b.step(total[k])
+            alone = b.download(cid)
+        for x, y in zip(alone, mixed[k]):
+            assert np.array_equal(x, y), k
+
+
+def test_c4_full_ensemble(nb, ora):
+    """The batched-ensemble workload at its stated shape: 512 Plummer clusters,
+    N_c log-uniform in [256, 32768], eps2 ~ U[1, 40], dt ~ U[0.001, 0.005], in
+    one batch. For the largest, the median and the smallest cluster: forces
+    against the FP64 oracle, and 5 steps in the mix equal to 5 steps alone, bit
+    for bit."""
+    seed, count = 4000, 512
+    sizes = nb.log_uniform_sizes(seed, count, 256, 32768)
+    eps2 = nb.uniform(seed + 1, count, 1.0, 40.0)
+    dts = nb.uniform(seed + 2, count, 0.001, 0.005)
+    cl = []
+    for c in range(count):
+        pos, vel, m = nb.ic_plummer(seed * 1000 + c, int(sizes[c]), 10.0, 0.1 * int(sizes[c]))
+        cl.append((pos, vel, m, float(np.float32(eps2[c])), float(np.float32(dts[c]))))
+    order = np.argsort(sizes, kind="stable")
+    pick = [int(order[-1]), int(order[count // 2]), int(order[0])]
+    print(f"C4: {int(sizes.sum())} bodies; sampled clusters of {[int(sizes[k]) for k in pick]}", flush=True)
+    with nb.Batch(sum((len(c[2]) + 255) // 256 * 256 for c in cl), count) as b:
+        ids = [b.add(p, v, m, e, d) for p, v, m, e, d in cl]
+        _, bodies, inter = b.info()
+        assert inter == sum(int(s) * (int(s) - 1) for s in sizes)
+        for k in pick:
+            p, v, m, e, d = cl[k]
+            f = b.forces(ids[k])
+            ref = ora.direct_f64(p.astype(np.float32).astype(np.float64), m.astype(np.float64), e)
+            err = rel_err(f, ref)
+            print(f"C4 cluster N={len(m)}: vs FP64 p50 {np.median(err):.2e} max {err.max():.2e}", flush=True)
+            assert np.median(err) < 1e-6 and np.percentile(err, 99) < 1e-5
+        b.step(5)
+        b.sync()
+        mixed = {k: b.download(ids[k]) for k in pick}
+    for k in pick:
+        p, v, m, e, d = cl[k]
+        with nb.Batch(len(m) + 256, 1) as b:
+            cid = b.add(p, v, m, e, d)
+            b.step(5)
+            alone = b.download(cid)
+        for x, y in zip(alone, mixed[k]):
+            assert np.array_equal(x, y), len(m)
