@@ -192,7 +192,7 @@ def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0, accuracy: boo
     log(f"bh: discs generated ({time.perf_counter() - t0:.1f} s)")
     with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH) as sim:
         sim.upload(pos, vel, m)
-        tuned = sim.tune_leaf(2)  # on these discs, not on a stand-in system
+        tuned = sim.tune_leaf(3)  # on these discs, not on a stand-in system
         log(f"bh: leaf size {tuned['leaf_size']} tuned on the discs ({tuned['bh_ms']:.2f} ms force pass)")
         inter = sim.count_interactions()
         inter_pb = sim.count_interactions_per_body()

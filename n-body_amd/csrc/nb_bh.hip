@@ -1222,7 +1222,7 @@ __device__ void traverse_range(const WaveLds& L, int lane, int P, float4 me, int
     lc = 0;                                                                                     \
   } while (0)
   // a virtual parent whose only child is the root
-  if (lane == 0) stack[0] = make_int2(0, (int)(uint32_t)n);
+  if (lane == 0) stack[0] = make_int2(0, (int)((1u << 27) | (uint32_t)n));
   __builtin_amdgcn_wave_barrier();
   int sp = 1, lc = 0;
   for (int iter = 0; sp > 0; ++iter) {
@@ -1238,18 +1238,20 @@ __device__ void traverse_range(const WaveLds& L, int lane, int P, float4 me, int
     float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
     float rr = 0.f;
     if (q < k) {
+      // entry: first child, and (slots << 27 | parent's body end); the slots stop
+      // at the end of the children's level, so every load below is in bounds and
+      // the three are issued together (a child is one whose bodies start before
+      // the parent's end)
       const int2 e = stack[sp + q];
       const uint32_t w = (uint32_t)e.y;
-      const int clev = (int)(w >> 27);
+      const int slots = (int)(w >> 27);
       const int pend = (int)(w & ((1u << 27) - 1));
-      const int cand = e.x + ci;
-      if (cand < L.lvl[clev + 1]) {
+      if (ci < slots) {
+        const int cand = e.x + ci;
         cl = link[cand];
-        if (cl.z < pend) {
-          child = cand;
-          c = com[cand];
-          rr = r2[cand];
-        }
+        c = com[cand];
+        rr = r2[cand];
+        if (cl.z < pend) child = cand;
       }
     }
     __builtin_amdgcn_wave_barrier();
@@ -1279,8 +1281,8 @@ __device__ void traverse_range(const WaveLds& L, int lane, int P, float4 me, int
     }
     if (open) {
       const uint32_t pend = (uint32_t)(cl.z + link_count(cl.w));
-      stack[sp + __popcll(bo & lt)] =
-          make_int2(cl.x, (int)(((uint32_t)(link_level(cl.w) + 1) << 27) | pend));
+      const int slots = max(0, min(8, L.lvl[link_level(cl.w) + 2] - cl.x));
+      stack[sp + __popcll(bo & lt)] = make_int2(cl.x, (int)(((uint32_t)slots << 27) | pend));
     }
     sp += __popcll(bo);
     // leaves that are not accepted -> the indices of their bodies

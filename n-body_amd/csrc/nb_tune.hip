@@ -162,28 +162,34 @@ extern "C" int nb_tune_leaf(nb_ctx* c, int32_t reps, nb_tuning* out) {
     if (e0) (void)hipEventDestroy(e0);
     return nb_fail(NB_E_HIP, "nb_tune_leaf: event creation failed");
   }
+  // Candidates are timed in interleaved rounds (one pass of each per round, the
+  // first round untimed) and each keeps its fastest pass, so clock and
+  // neighbour drift during the sweep affects every candidate alike.
+  constexpr int kLeaves[] = {1, 2, 4, 8, 16, 32};
+  constexpr int kCount = sizeof(kLeaves) / sizeof(kLeaves[0]);
+  double ms[kCount];
+  for (double& m : ms) m = DBL_MAX;
   int rc = NB_OK, best_leaf = c->cfg.leaf_size;
   double best_ms = DBL_MAX;
-  for (int leaf : {1, 2, 4, 8, 16, 32}) {
-    c->cfg.leaf_size = leaf;
-    for (int r = -1; !rc && r < reps; ++r) {  // one warm-up pass
-      if (r == 0) (void)hipEventRecord(e0, c->stream);
+  for (int r = -1; !rc && r < reps; ++r) {
+    for (int k = 0; !rc && k < kCount; ++k) {
+      c->cfg.leaf_size = kLeaves[k];
+      (void)hipEventRecord(e0, c->stream);
       hipError_t e = nb_bh_build(c);
       if (e == hipSuccess) e = nb_bh_traverse(c, false);
+      if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+      float t = 0.f;
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
       if (e != hipSuccess) rc = nb_fail(NB_E_HIP, std::string("nb_tune_leaf: ") + hipGetErrorString(e));
-    }
-    if (rc) break;
-    (void)hipEventRecord(e1, c->stream);
-    float t = 0.f;
-    if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
-      rc = nb_fail(NB_E_HIP, "nb_tune_leaf: timing failed");
-      break;
-    }
-    if (t / reps < best_ms) {
-      best_ms = t / reps;
-      best_leaf = leaf;
+      else if (r >= 0 && t < ms[k]) ms[k] = t;
     }
   }
+  for (int k = 0; !rc && k < kCount; ++k)
+    if (ms[k] < best_ms) {
+      best_ms = ms[k];
+      best_leaf = kLeaves[k];
+    }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c->cfg.leaf_size = best_leaf;
