@@ -63,6 +63,56 @@ constexpr int kWaveBodies = NB_BH_WAVE_BODIES;
 static_assert(kWaveBodies >= 4 && kWaveBodies <= 64 && (kWaveBodies & (kWaveBodies - 1)) == 0,
               "a wave range fits one wave");
 
+// ---- 0. wave64 prefix sums on DPP ----------------------------------------------------
+// Inclusive prefix over the 64 lanes of a wave, entirely in cross-lane DPP adds:
+// shifts by 1, 2, 4 and 8 lanes inside each row of 16 (row_shr; a lane whose
+// source lies outside its row adds 0), then lane 15 of rows 0 and 2 is added to
+// rows 1 and 3 (row_bcast:15) and lane 31 to rows 2 and 3 (row_bcast:31). Six
+// adds and no LDS round trips; the association is fixed, so FP64 prefixes are
+// deterministic.
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ int dpp_shift(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, kCtrl, kRowMask, 0xf, false);
+}
+template <int kCtrl, int kRowMask>
+__device__ __forceinline__ double dpp_shift(double v) {  // the two halves moved alike
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint32_t lo =
+      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, kCtrl, kRowMask, 0xf, false);
+  const uint32_t hi =
+      (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), kCtrl, kRowMask, 0xf, false);
+  return __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+}
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T v) {
+  v += dpp_shift<0x111, 0xf>(v);  // row_shr:1
+  v += dpp_shift<0x112, 0xf>(v);  // row_shr:2
+  v += dpp_shift<0x114, 0xf>(v);  // row_shr:4
+  v += dpp_shift<0x118, 0xf>(v);  // row_shr:8
+  v += dpp_shift<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_shift<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
+// Exclusive prefix of v over the threads of a kScanThreads-thread block (wave
+// prefixes, then the wave totals added in wave order); total = the block's sum.
+// wsum: kScanThreads / 64 ints of LDS, free again when this returns.
+__device__ __forceinline__ int block_excl_scan(int v, int* wsum, int& total) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int incl = wave_incl_scan(v);
+  if (lane == 63) wsum[wave] = incl;
+  __syncthreads();
+  int before = 0;
+  total = 0;
+#pragma unroll
+  for (int w = 0; w < kScanThreads / 64; ++w) {
+    before += w < wave ? wsum[w] : 0;
+    total += wsum[w];
+  }
+  __syncthreads();
+  return before + incl - v;
+}
+
 // ---- 1. bounding box --------------------------------------------------------
 __global__ __launch_bounds__(256) void k_bbox_partial(const float4* __restrict__ pm, int n,
                                                       float* __restrict__ part) {
@@ -192,6 +242,7 @@ __device__ __forceinline__ int scan_count(const int* level_off, int l, int n_con
   return c < 0 ? 0 : (c > cap ? cap : c);
 }
 
+// Pass 1 of an exclusive int scan: the sum of every kScanTile-element tile.
 __global__ __launch_bounds__(kScanThreads) void k_iscan_reduce(const int* __restrict__ in,
                                                                const int* level_off, int l,
                                                                int n_const, int cap,
@@ -203,53 +254,42 @@ __global__ __launch_bounds__(kScanThreads) void k_iscan_reduce(const int* __rest
     const int e = base + k;
     if (e < n) s += in[e];
   }
-  __shared__ int red[kScanThreads];
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = kScanThreads / 2; w > 0; w >>= 1) {
-    if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+  __shared__ int wsum[kScanThreads / 64];
+  int tile = 0;
+  (void)block_excl_scan(s, wsum, tile);
+  if (threadIdx.x == 0) part[blockIdx.x] = tile;
 }
 
-// Exclusive scan of nparts block sums by one block; writes the grand total to
-// part[nparts]. When level_off is given it also publishes the next level's end:
+// Pass 2: every tile adds up the tile sums before it (a few KB of L2 reads per
+// tile, which replaces a separate one-block scan of the sums and its launch),
+// then rescans its own elements from there (8 consecutive elements per thread,
+// one block scan of the thread sums). Block 0 also adds up all nparts tile sums
+// into part[nparts] and, for a level, publishes the next level's end:
 // level_off[l + 2] = level_off[l + 1] + total.
-__global__ __launch_bounds__(kScanThreads) void k_iscan_partials(int* __restrict__ part,
-                                                                 int nparts, int* level_off,
-                                                                 int l) {
-  __shared__ int sh[kScanThreads];
-  int carry = 0;
-  for (int b0 = 0; b0 < nparts; b0 += kScanThreads) {
-    const int b = b0 + threadIdx.x;
-    const int v = b < nparts ? part[b] : 0;
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int d = 1; d < kScanThreads; d <<= 1) {
-      const int t = threadIdx.x >= d ? sh[threadIdx.x - d] : 0;
-      __syncthreads();
-      sh[threadIdx.x] += t;
-      __syncthreads();
-    }
-    if (b < nparts) part[b] = carry + sh[threadIdx.x] - v;
-    carry += sh[kScanThreads - 1];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    part[nparts] = carry;
-    if (level_off) level_off[l + 2] = level_off[l + 1] + carry;
-  }
-}
-
 __global__ __launch_bounds__(kScanThreads) void k_iscan_final(const int* __restrict__ in,
                                                               const int* level_off, int l,
                                                               int n_const, int cap,
-                                                              const int* __restrict__ part,
+                                                              int* __restrict__ part, int nparts,
+                                                              int* level_off_mut,
                                                               int* __restrict__ out) {
   const int n = scan_count(level_off, l, n_const, cap);
   const int base = blockIdx.x * kScanTile;
+  __shared__ int wsum[kScanThreads / 64];
+  if (blockIdx.x == 0) {
+    int all = 0;
+    for (int b = threadIdx.x; b < nparts; b += kScanThreads) all += part[b];
+    int total = 0;
+    (void)block_excl_scan(all, wsum, total);
+    if (threadIdx.x == 0) {
+      part[nparts] = total;
+      if (level_off_mut) level_off_mut[l + 2] = level_off[l + 1] + total;
+    }
+  }
   if (base >= n) return;
+  int pre = 0;
+  for (int b = threadIdx.x; b < (int)blockIdx.x; b += kScanThreads) pre += part[b];
+  int start = 0;
+  (void)block_excl_scan(pre, wsum, start);
   constexpr int kPer = kScanTile / kScanThreads;  // 8 consecutive elements per thread
   int v[kPer];
   int s = 0;
@@ -259,16 +299,8 @@ __global__ __launch_bounds__(kScanThreads) void k_iscan_final(const int* __restr
     v[k] = e < n ? in[e] : 0;
     s += v[k];
   }
-  __shared__ int sh[kScanThreads];
-  sh[threadIdx.x] = s;
-  __syncthreads();
-  for (int d = 1; d < kScanThreads; d <<= 1) {
-    const int t = threadIdx.x >= d ? sh[threadIdx.x - d] : 0;
-    __syncthreads();
-    sh[threadIdx.x] += t;
-    __syncthreads();
-  }
-  int run = part[blockIdx.x] + sh[threadIdx.x] - s;
+  int total = 0;
+  int run = start + block_excl_scan(s, wsum, total);
 #pragma unroll
   for (int k = 0; k < kPer; ++k) {
     const int e = base + threadIdx.x * kPer + k;
@@ -278,8 +310,8 @@ __global__ __launch_bounds__(kScanThreads) void k_iscan_final(const int* __restr
 }
 
 // Exclusive scan of at most kSingleScan elements by one workgroup (8 consecutive
-// elements per thread), for the small scans of small systems: one launch
-// instead of three. Writes level_off[l + 2] like k_iscan_partials.
+// elements per thread), for the small scans: one launch instead of two. Writes
+// level_off[l + 2] like k_iscan_final.
 constexpr int kSingleScan = 8192;
 __global__ __launch_bounds__(1024) void k_iscan_single(const int* __restrict__ in,
                                                        const int* level_off, int l, int n_const,
@@ -296,12 +328,7 @@ __global__ __launch_bounds__(1024) void k_iscan_single(const int* __restrict__ i
     sum += v[k];
   }
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  int incl = sum;
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    const int t = __shfl_up(incl, off);
-    if (lane >= off) incl += t;
-  }
+  const int incl = wave_incl_scan(sum);
   __shared__ int wsum[16];
   if (lane == 63) wsum[wave] = incl;
   __syncthreads();
@@ -351,41 +378,48 @@ __global__ __launch_bounds__(kScanThreads) void k_mscan_reduce(const float4* __r
   if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 
+__device__ __forceinline__ double4 shfl_up4(double4 v, int off) {
+  return make_double4(__shfl_up(v.x, off), __shfl_up(v.y, off), __shfl_up(v.z, off),
+                      __shfl_up(v.w, off));
+}
+__device__ __forceinline__ double4 wave_incl_scan4(double4 v) {
+  return make_double4(wave_incl_scan(v.x), wave_incl_scan(v.y), wave_incl_scan(v.z),
+                      wave_incl_scan(v.w));
+}
+
 // Exclusive scan of the block sums by one block, in a fixed order: each thread
-// folds a contiguous run of partials, a Hillis-Steele pass scans the 256 run
-// totals, then each thread rewrites its run. part[nparts] receives the total.
+// folds a contiguous run of partials, the 256 run totals are scanned (wave DPP
+// prefixes, then the 4 wave totals in wave order), then each thread rewrites its
+// run. part[nparts] receives the total.
 __global__ __launch_bounds__(kScanThreads) void k_mscan_partials(double4* __restrict__ part,
                                                                  int nparts) {
   const int per = (nparts + kScanThreads - 1) / kScanThreads;
-  const int b0 = threadIdx.x * per;
+  const int b0 = min((int)threadIdx.x * per, nparts);
   const int b1 = min(b0 + per, nparts);
   double4 tot = make_double4(0, 0, 0, 0);
   for (int b = b0; b < b1; ++b) tot = add4(tot, part[b]);
-  __shared__ double4 sh[kScanThreads];
-  sh[threadIdx.x] = tot;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double4 incl = wave_incl_scan4(tot);
+  __shared__ double4 wtot[kScanThreads / 64];
+  if (lane == 63) wtot[wave] = incl;
   __syncthreads();
-  for (int d = 1; d < kScanThreads; d <<= 1) {
-    const double4 t = threadIdx.x >= d ? sh[threadIdx.x - d] : make_double4(0, 0, 0, 0);
-    __syncthreads();
-    sh[threadIdx.x] = add4(sh[threadIdx.x], t);
-    __syncthreads();
+  double4 run = make_double4(0, 0, 0, 0), grand = make_double4(0, 0, 0, 0);
+  for (int w = 0; w < kScanThreads / 64; ++w) {
+    if (w < wave) run = add4(run, wtot[w]);
+    grand = add4(grand, wtot[w]);
   }
-  double4 run = threadIdx.x > 0 ? sh[threadIdx.x - 1] : make_double4(0, 0, 0, 0);
+  const double4 below = shfl_up4(incl, 1);  // inclusive value of the lane below
+  if (lane > 0) run = add4(run, below);
   for (int b = b0; b < b1; ++b) {
     const double4 v = part[b];
     part[b] = run;
     run = add4(run, v);
   }
-  if (threadIdx.x == kScanThreads - 1) part[nparts] = sh[kScanThreads - 1];
-}
-
-__device__ __forceinline__ double4 shfl_up4(double4 v, int off) {
-  return make_double4(__shfl_up(v.x, off), __shfl_up(v.y, off), __shfl_up(v.z, off),
-                      __shfl_up(v.w, off));
+  if (threadIdx.x == 0) part[nparts] = grand;
 }
 
 // Exclusive prefix of each tile, row by row: a row of 256 bodies is scanned by
-// wave shuffles and a fixed-order combine of the 4 wave totals; rows follow
+// wave DPP prefixes and a fixed-order combine of the 4 wave totals; rows follow
 // each other in order. Deterministic (no atomics, fixed association).
 __global__ __launch_bounds__(kScanThreads) void k_mscan_final(const float4* __restrict__ pms,
                                                               int n,
@@ -400,12 +434,8 @@ __global__ __launch_bounds__(kScanThreads) void k_mscan_final(const float4* __re
 #pragma unroll
   for (int r = 0; r < kRows; ++r) {
     const int e = base + r * kScanThreads + threadIdx.x;
-    double4 x = e < n ? body_moments(pms[e]) : make_double4(0, 0, 0, 0);
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const double4 t = shfl_up4(x, off);
-      if (lane >= off) x = add4(x, t);
-    }
+    const double4 x0 = e < n ? body_moments(pms[e]) : make_double4(0, 0, 0, 0);
+    const double4 x = wave_incl_scan4(x0);
     if (lane == 63) wtot[r][wave] = x;
     const double4 below = shfl_up4(x, 1);  // inclusive value of the lane below
     incl[r] = lane > 0 ? below : make_double4(0, 0, 0, 0);
@@ -494,12 +524,7 @@ __global__ __launch_bounds__(kSortThreads) void k_radix_scatter(
     // tile-local digit starts: exclusive scan of the tile's digit counts
     const int d = threadIdx.x;
     const int cnt = whist[0][d] + whist[1][d] + whist[2][d] + whist[3][d];
-    int incl = cnt;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int t = __shfl_up(incl, off);
-      if (lane >= off) incl += t;
-    }
+    const int incl = wave_incl_scan(cnt);
     __shared__ int wsum[4];
     if (lane == 63) wsum[wave] = incl;
     __syncthreads();
@@ -749,12 +774,7 @@ __global__ __launch_bounds__(256) void k_tree_emit(const uint32_t* __restrict__ 
     // one atomic per wave reserves the slots of all its lanes
     const bool packs = internal && link_count(nd.w) > kWaveBodies;
     const int nr = packs ? pack_ranges(starts, ends, nc, l, leaf, [](int, int, int) {}) : 0;
-    int incl = nr;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-      const int v = __shfl_up(incl, off);
-      if (lane >= off) incl += v;
-    }
+    const int incl = wave_incl_scan(nr);
     const int total = __shfl(incl, 63);
     int wbase = 0;
     if (lane == 63 && total > 0) wbase = atomicAdd(&wctl[0], total);
@@ -1398,10 +1418,8 @@ hipError_t iscan(const int* in, int* out, int* part, const int* level_off, int l
   }
   hipLaunchKernelGGL(k_iscan_reduce, dim3(blocks), dim3(kScanThreads), 0, s, in, level_off, l,
                      n_const, bound, part);
-  hipLaunchKernelGGL(k_iscan_partials, dim3(1), dim3(kScanThreads), 0, s, part, blocks,
-                     level_off_mut, l);
   hipLaunchKernelGGL(k_iscan_final, dim3(blocks), dim3(kScanThreads), 0, s, in, level_off, l,
-                     n_const, bound, part, out);
+                     n_const, bound, part, blocks, level_off_mut, out);
   return hipGetLastError();
 }
 
