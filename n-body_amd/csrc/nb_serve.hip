@@ -467,19 +467,23 @@ int nb_server_tick(nb_server* s, int32_t* running_out, int32_t* finished_out) {
   // 1. select among the first `lookahead` queued jobs, in arrival order or
   //    smallest estimated total work first (policy)
   {
-    std::vector<int64_t> cand;
-    for (auto it = s->queue.begin(); it != s->queue.end() && (int)cand.size() < s->cfg.lookahead; ++it)
-      cand.push_back(*it);
-    if (s->cfg.policy == NB_SERVE_SHORTEST)
-      std::stable_sort(cand.begin(), cand.end(), [&](int64_t a, int64_t b) {
-        const Job& x = *s->jobs.at(a);
-        const Job& y = *s->jobs.at(b);
-        return x.cost_ms * x.desc.steps < y.cost_ms * y.desc.steps;
-      });
+    // (sort key, job): the key is the estimated total work under shortest-first
+    // and 0 in arrival order; ids rise with arrival, so equal keys keep it. One
+    // map lookup per candidate, not one per comparison (4096 candidates a tick).
+    std::vector<std::pair<std::pair<double, int64_t>, Job*>> cand;
+    const bool shortest = s->cfg.policy == NB_SERVE_SHORTEST;
+    for (auto it = s->queue.begin(); it != s->queue.end() && (int)cand.size() < s->cfg.lookahead; ++it) {
+      Job* j = s->jobs.at(*it).get();
+      cand.push_back({{shortest ? j->cost_ms * j->desc.steps : 0.0, *it}, j});
+    }
+    if (shortest)
+      std::sort(cand.begin(), cand.end(),
+                [](const auto& a, const auto& b) { return a.first < b.first; });
     std::vector<int64_t> taken;
-    for (int64_t id : cand) {
+    for (const auto& c : cand) {
       if ((int)(s->running.size() + s->preparing.size()) >= s->cfg.max_jobs) break;
-      Job& j = *s->jobs.at(id);
+      const int64_t id = c.first.second;
+      Job& j = *c.second;
       const bool idle = s->running.empty() && s->preparing.empty();
       // shortest-first oversubscribes: admitted jobs beyond one tick's budget
       // wait resident (no steps) while jobs with less remaining work run
