@@ -75,6 +75,13 @@ def test_argument_errors(nb):
                  (dummy, 100, 1.0, 1.0, 0.5, 8, nb.F_NO_GRAPH, dummy, None, None)):
         assert lib.nb_bh_forces_noalloc(*args) == nb.NB_E_ARG, args
         assert lib.nb_last_error()
+    # entry points added for diagnostics, tuning, quotas and serving statistics
+    # reject null handles before any device call
+    assert lib.nb_count_interactions_per_body(None, None) == nb.NB_E_ARG
+    assert lib.nb_direct_f64(None, None, 1, None) == nb.NB_E_ARG
+    assert lib.nb_tune_leaf(None, 1, None) == nb.NB_E_ARG
+    assert lib.nb_batch_set_quotas(None, None, None, 1) == nb.NB_E_ARG
+    assert lib.nb_server_get_stats(None, None) == nb.NB_E_ARG
 
 
 def test_initial_conditions_are_seeded(nb):

@@ -159,6 +159,31 @@ def test_pooled_context_serves_the_next_job(nb):
         assert np.array_equal(rp, p) and np.array_equal(rv, v)
 
 
+def test_loop_statistics(nb):
+    """nb_server_get_stats: ticks and steps are counted (the steps of finished
+    jobs, no more), finished Barnes-Hut contexts wait in the pool, and the
+    loop's phases add up to no more than the wall time spent in ticks."""
+    jobs = [(1500, 12, 0, 61), (9000, 5, 1, 62), (3000, 7, 0, 63), (12000, 4, 0, 64)]
+    with nb.Server(bh_threshold=8192, direct_pool=1 << 16) as s:
+        ids = [s.submit(*j) for j in jobs]
+        t0 = time.monotonic()
+        wall = 0.0
+        while time.monotonic() - t0 < 60:
+            t1 = time.perf_counter()
+            s.tick()
+            wall += time.perf_counter() - t1
+            if all(s.status(i)["state"] == nb.JOB_DONE for i in ids):
+                break
+        st = s.stats()
+        assert all(s.status(i)["state"] == nb.JOB_DONE for i in ids)
+    assert st["ticks"] >= 1
+    assert st["steps"] == sum(j[1] for j in jobs)
+    assert st["pooled_contexts"] >= 1  # the Barnes-Hut jobs' contexts went back to the pool
+    phases = [st[k] for k in ("select_ms", "idle_wait_ms", "start_ms", "enqueue_ms",
+                              "device_wait_ms", "retire_ms")]
+    assert min(phases) >= 0.0 and sum(phases) <= 1e3 * wall * 1.01 + 1.0
+
+
 def poisson_stream(seed, rate, duration):
     """The serving workload's stream: Poisson arrivals, N log-uniform in
     [1024, 2^20], steps uniform in [10, 500], Plummer or ball with p = 1/2."""
