@@ -254,8 +254,8 @@ typedef struct nb_batch_config {
   int64_t max_bodies;      /* body pool capacity */
   int32_t max_clusters;
   int32_t integrator;      /* nb_integrator, shared by all clusters */
-  int32_t bodies_per_lane; /* 0 = built-in 2 (nb_batch_tune picks per ensemble) */
-  int32_t j_split;         /* 0 = built-in 8 (4 with bodies_per_lane 4) */
+  int32_t bodies_per_lane; /* 0 = built-in 1 (nb_batch_tune picks per ensemble) */
+  int32_t j_split;         /* 0 = built-in 4 */
   int32_t flags;           /* NB_F_NO_GRAPH */
   int32_t pad_;
 } nb_batch_config;

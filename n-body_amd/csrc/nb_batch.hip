@@ -9,9 +9,10 @@
 //
 // Work decomposition:
 //   force      work item = (slot, i-block of 64*B bodies); items are sorted by
-//              cluster size, largest first, and a fixed grid of workgroups pulls
-//              them from an atomic ticket (persistent kernel; every workgroup
-//              exits once the ticket passes the item count). The K waves of a
+//              cluster size, largest first, and as many workgroups as can be
+//              resident at once pull them from an atomic ticket (persistent
+//              kernel; every workgroup exits once the ticket passes the item
+//              count). The K waves of a
 //              workgroup split the cluster's sources; partial sums merge in LDS
 //              in wave order, so a cluster's forces do not depend on scheduling
 //              or on the other clusters.
@@ -32,8 +33,8 @@ struct nb_batch {
   nb_batch_config cfg{};
   hipStream_t stream = nullptr;
   int64_t cap = 0;
-  int bpl = 2, jsplit = 8;
-  int cus = 256;  // compute units: the persistent grid is 16 waves per CU
+  int bpl = 1, jsplit = 4;
+  int cus = 256;  // compute units: the persistent grid fills every CU to its occupancy
   // device pool
   float4* pm = nullptr;
   double *pos = nullptr, *vel = nullptr, *acc = nullptr, *force = nullptr;
@@ -220,7 +221,20 @@ __global__ __launch_bounds__(256) void k_batch_energy(int off, int n, int64_t ca
 
 template <int B, int K>
 hipError_t launch_force(nb_batch* b, int use_quota) {
-  const int grid = b->cus * 16 / K;  // 16 waves per CU in flight
+  // Persistent: as many blocks as can be resident at once, which depends on the
+  // shape's registers and LDS (one body per lane fits 8 waves per SIMD, two
+  // bodies 4). A fixed 16 waves per CU left the one-body shapes at half
+  // occupancy: 1x4 took 16.08 ms on the 512-cluster ensemble, 14.37 ms at full
+  // occupancy.
+  static const int per_cu = [] {
+    int nbk = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbk, k_batch_force<B, K>, 64 * K, 0) !=
+            hipSuccess ||
+        nbk < 1)
+      nbk = std::max(1, 16 / K);
+    return nbk;
+  }();
+  const int grid = b->cus * per_cu;
   hipLaunchKernelGGL((k_batch_force<B, K>), dim3(grid), dim3(64 * K), 0, b->stream, b->pm,
                      b->c_off, b->c_n, b->c_npad, b->c_eps2, b->c_G, b->c_quota, use_quota,
                      b->items, b->counts, b->cap, b->force);
@@ -356,11 +370,12 @@ int nb_batch_create(nb_batch** out, const nb_batch_config* cfg) {
   b->cfg = *cfg;
   b->cap = (cfg->max_bodies + 255) / 256 * 256;
   // Built-in shape, measured on MI355X on the 512-cluster ensemble
-  // (N_c log-uniform in [256, 32768]): two bodies per lane and eight waves per
-  // work item, 16.7 ms/step against 18.0 ms for one and four. nb_batch_tune
-  // times every shape on the batch's own clusters.
-  b->bpl = cfg->bodies_per_lane > 0 ? cfg->bodies_per_lane : 2;
-  b->jsplit = cfg->j_split > 0 ? cfg->j_split : (b->bpl == 4 ? 4 : 8);
+  // (N_c log-uniform in [256, 32768]) with occupancy-sized launches: one body
+  // per lane and four waves per work item, 14.4 ms per force pass against
+  // 14.7 ms for 1x8 and 15.4 ms for 2x4. nb_batch_tune times every shape on the
+  // batch's own clusters.
+  b->bpl = cfg->bodies_per_lane > 0 ? cfg->bodies_per_lane : 1;
+  b->jsplit = cfg->j_split > 0 ? cfg->j_split : 4;
   int dev = 0, cus = 256;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
