@@ -226,14 +226,13 @@ hipError_t launch_force(nb_batch* b, int use_quota) {
   // bodies 4). A fixed 16 waves per CU left the one-body shapes at half
   // occupancy: 1x4 took 16.08 ms on the 512-cluster ensemble, 14.37 ms at full
   // occupancy.
-  static const int per_cu = [] {
-    int nbk = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nbk, k_batch_force<B, K>, 64 * K, 0) !=
-            hipSuccess ||
-        nbk < 1)
-      nbk = std::max(1, 16 / K);
-    return nbk;
-  }();
+  // (Queried per launch, for the current device: a host-side query, and steps
+  // replay a captured graph, so it runs once per capture.)
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_batch_force<B, K>, 64 * K, 0) !=
+          hipSuccess ||
+      per_cu < 1)
+    per_cu = std::max(1, 16 / K);
   const int grid = b->cus * per_cu;
   hipLaunchKernelGGL((k_batch_force<B, K>), dim3(grid), dim3(64 * K), 0, b->stream, b->pm,
                      b->c_off, b->c_n, b->c_npad, b->c_eps2, b->c_G, b->c_quota, use_quota,
