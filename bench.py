@@ -343,7 +343,7 @@ def bench_batch(nb, steps: int, warmup: int, energy: bool, tune: bool = True, cp
             "interactions_per_s": inter * steps / wall,
             "roofline": valu_roofline("k_batch_force", inter, t["force_ms"] / max(1, t["force_calls"])),
             "shape": {"bodies_per_lane": shape["bodies_per_lane"], "j_split": shape["j_split"],
-                      "tuned": True} if shape else "built-in 2x8",
+                      "tuned": True} if shape else "built-in 1x4",
         }
         if energy:
             e1 = [b.energy(i) for i in ids]
