@@ -119,7 +119,9 @@ def test_theta_tiny_is_direct_sum(nb, ora):
     assert rel_err(f, ref).max() < 1e-5
 
 
-@pytest.mark.parametrize("n", [1, 2, 5, 4097])
+# 8193 and 65537 put the level scans and then the radix-histogram scans just
+# past one workgroup's 8192 elements (two launches with a nearly empty tile)
+@pytest.mark.parametrize("n", [1, 2, 5, 4097, 8193, 65537])
 def test_small_and_ragged(nb, ora, n):
     pos, vel, m = nb.ic_ball(n, n, 10.0, 1.0)
     f, codes, perm, nodes, bbox, inter = gpu_run(nb, pos, vel, m, 4)
