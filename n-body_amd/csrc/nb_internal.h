@@ -213,9 +213,11 @@ struct nb_ctx {
   bool uploaded = false;
   bool tree_valid = false;
 
-  // one-step hipGraph
+  // captured step graph: graph_steps consecutive steps (more than one for small
+  // direct-sum systems, whose steps are short enough for the graph launch to show)
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  int graph_steps = 1;
 
   // timing
   bool timing = false;

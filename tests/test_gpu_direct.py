@@ -158,12 +158,18 @@ def test_euler_integrator(nb, ora):
     assert rel_err(v, ov).max() < 1e-4
 
 
-def test_graph_replay_matches_eager(nb):
+@pytest.mark.parametrize("cases", [
+    ((0, [10]), (1, [1] * 10), (0, [3, 3, 4])),
+    # small systems replay eight steps per graph: whole graphs, remainders run
+    # eagerly, and any split of the steps gives the same bits
+    ((0, [26]), (1, [26]), (0, [17, 1, 8]), (0, [7, 9, 2, 8])),
+])
+def test_graph_replay_matches_eager(nb, cases):
     n = 2048
     pos, vel, m = nb.ic_plummer(10, n, 10.0, 100.0)
     out = []
-    for flags, chunks in ((0, [10]), (nb.F_NO_GRAPH, [1] * 10), (0, [3, 3, 4])):
-        with nb.Simulation(n, 1.0, 0.001, flags=flags) as sim:
+    for eager, chunks in cases:
+        with nb.Simulation(n, 1.0, 0.001, flags=nb.F_NO_GRAPH if eager else 0) as sim:
             sim.upload(pos, vel, m)
             for k in chunks:
                 sim.step(k)
