@@ -36,6 +36,7 @@
 //                  a leaf that is not accepted is evaluated body by body.
 #include <algorithm>
 #include <cfloat>
+#include <type_traits>
 #include <vector>
 
 #include "nb_force_core.h"
@@ -60,8 +61,8 @@ constexpr int kScanThreads = 256;
 // fewer interactions and a 4% faster traversal on the 4M two-disc system (16
 // bodies: 14% slower, the walks dominate).
 constexpr int kWaveBodies = NB_BH_WAVE_BODIES;
-static_assert(kWaveBodies >= 4 && kWaveBodies <= 64 && (kWaveBodies & (kWaveBodies - 1)) == 0,
-              "a wave range fits one wave");
+static_assert(kWaveBodies >= 4 && kWaveBodies <= 32 && (kWaveBodies & (kWaveBodies - 1)) == 0,
+              "two wave ranges fit one wave");
 
 // ---- 0. wave64 prefix sums on DPP ----------------------------------------------------
 // Inclusive prefix over the 64 lanes of a wave, entirely in cross-lane DPP adds:
@@ -719,6 +720,59 @@ __device__ __forceinline__ int pack_ranges(const int* starts, const int* ends, i
   return r;
 }
 
+// P lanes per copy of a range of `count` bodies: a power of two >= count, >= 4.
+__device__ __forceinline__ int copy_lanes(int count) {
+  return count <= 4 ? 4 : 1 << (32 - __clz(count - 1));
+}
+
+// Number of records emit_records makes of one node's wave ranges.
+__device__ __forceinline__ int count_records(const int* starts, const int* ends, int nc, int l,
+                                             int leaf) {
+  int per_width[4] = {0, 0, 0, 0};  // ranges of width 4, 8, 16, 32
+  pack_ranges(starts, ends, nc, l, leaf, [&](int, int, int c) {
+    const int k = 29 - __clz(copy_lanes(c));
+#pragma unroll
+    for (int j = 0; j < 4; ++j) per_width[j] += j == k ? 1 : 0;
+  });
+  return ((per_width[0] + 1) >> 1) + ((per_width[1] + 1) >> 1) + ((per_width[2] + 1) >> 1) +
+         ((per_width[3] + 1) >> 1);
+}
+
+// One node's wave ranges (pack_ranges) as records of two sibling ranges of the
+// same copy width (a pair shares one lane layout, so neither range idles
+// lanes), (start, count, start, count), or of one range, (start, count, 0, 0):
+// each range waits for the next of its width, in order. Writes count_records()
+// records from record `first` on. There are at most as many records as ranges,
+// and at most n ranges.
+__device__ __forceinline__ void emit_records(int4* __restrict__ waves, int first, int n,
+                                             const int* starts, const int* ends, int nc, int l,
+                                             int leaf) {
+  int2 wait[4];  // one waiting range per width 4, 8, 16, 32
+#pragma unroll
+  for (int k = 0; k < 4; ++k) wait[k] = make_int2(0, 0);
+  int r = 0;
+  auto put = [&](int4 v) {
+    if (first + r < n) waves[first + r] = v;
+    ++r;
+  };
+  pack_ranges(starts, ends, nc, l, leaf, [&](int, int a, int c) {
+    const int k = 29 - __clz(copy_lanes(c));  // 4 -> 0, ..., 32 -> 3
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      if (j == k) {
+        if (wait[j].y > 0) {
+          put(make_int4(wait[j].x, wait[j].y, a, c));
+          wait[j] = make_int2(0, 0);
+        } else {
+          wait[j] = make_int2(a, c);
+        }
+      }
+  });
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (wait[j].y > 0) put(make_int4(wait[j].x, wait[j].y, 0, 0));
+}
+
 __global__ __launch_bounds__(256) void k_tree_emit(const uint32_t* __restrict__ codes, int n,
                                                    int4* __restrict__ link,
                                                    const int* __restrict__ level_off, int l,
@@ -726,7 +780,7 @@ __global__ __launch_bounds__(256) void k_tree_emit(const uint32_t* __restrict__ 
                                                    const int* __restrict__ child_off,
                                                    const int* __restrict__ child_bounds,
                                                    int64_t max_nodes, int* __restrict__ flags,
-                                                   int2* __restrict__ waves,
+                                                   int4* __restrict__ waves,
                                                    int* __restrict__ wctl) {
   const int lo = level_off[l];
   const int hi = min(level_off[l + 1], lo + cap);
@@ -770,22 +824,16 @@ __global__ __launch_bounds__(256) void k_tree_emit(const uint32_t* __restrict__ 
         link[k].x = first;
       }
     }
-    // wave ranges of the children of nodes holding more than kWaveBodies bodies:
-    // one atomic per wave reserves the slots of all its lanes
+    // wave ranges of the children of nodes holding more than kWaveBodies bodies,
+    // in records of two: one atomic per wave reserves the records of all its lanes
     const bool packs = internal && link_count(nd.w) > kWaveBodies;
-    const int nr = packs ? pack_ranges(starts, ends, nc, l, leaf, [](int, int, int) {}) : 0;
-    const int incl = wave_incl_scan(nr);
+    const int nrec = packs ? count_records(starts, ends, nc, l, leaf) : 0;
+    const int incl = wave_incl_scan(nrec);
     const int total = __shfl(incl, 63);
     int wbase = 0;
     if (lane == 63 && total > 0) wbase = atomicAdd(&wctl[0], total);
     wbase = __shfl(wbase, 63);
-    if (nr > 0) {
-      const int first_slot = wbase + incl - nr;
-      pack_ranges(starts, ends, nc, l, leaf, [&](int r, int a, int c) {
-        const int slot = first_slot + r;
-        if (slot < n) waves[slot] = make_int2(a, c);  // at most n ranges exist
-      });
-    }
+    if (nrec > 0) emit_records(waves, wbase + incl - nrec, n, starts, ends, nc, l, leaf);
   }
 }
 
@@ -833,8 +881,9 @@ __global__ __launch_bounds__(256) void k_top_bounds(const uint32_t* __restrict__
 __global__ __launch_bounds__(1024) void k_top_build(const int* __restrict__ top_start, int n,
                                                     int leaf, int4* __restrict__ link,
                                                     int* __restrict__ level_off,
-                                                    int2* __restrict__ waves,
-                                                    int* __restrict__ wctl) {
+                                                    int4* __restrict__ waves,
+                                                    int* __restrict__ wctl,
+                                                    int* __restrict__ rec_hist) {
   __shared__ int s_idx[kTopCells];   // exclusive scan of existence, then node index
   __shared__ int s_next[kTopCells];  // rope of each node
   __shared__ int s_part[1024];
@@ -884,15 +933,25 @@ __global__ __launch_bounds__(1024) void k_top_build(const int* __restrict__ top_
   }
   if (tid == 1023) level_off[kTopLevels + 1] = s_part[1023];
   __syncthreads();
+  if (tid < kCostBuckets) rec_hist[tid] = 0;  // this pass's traversal order starts empty
   if (tid == 0) {
     wctl[1] = 0;  // traversal ticket
     if (n <= kWaveBodies) {
-      waves[0] = make_int2(0, n);
+      waves[0] = make_int4(0, n, 0, 0);
       s_ranges = 1;
     } else if (n <= leaf) {
-      // the root is itself a leaf, so no node packs ranges below it: chunk it
+      // the root is itself a leaf, so no node packs ranges below it: chunk it,
+      // two full chunks per record, a shorter last chunk on its own
       int r = 0;
-      for (int o = 0; o < n; o += kWaveBodies) waves[r++] = make_int2(o, min(kWaveBodies, n - o));
+      for (int o = 0; o < n; o += kWaveBodies) {
+        const int k = min(kWaveBodies, n - o), o2 = o + kWaveBodies;
+        if (k == kWaveBodies && n - o2 >= kWaveBodies) {
+          waves[r++] = make_int4(o, k, o2, kWaveBodies);
+          o = o2;
+        } else {
+          waves[r++] = make_int4(o, k, 0, 0);
+        }
+      }
       s_ranges = r;
     }
   }
@@ -933,10 +992,8 @@ __global__ __launch_bounds__(1024) void k_top_build(const int* __restrict__ top_
             ends[nc] = top_start[eb + d + 1];
             ++nc;
           }
-        pack_ranges(starts, ends, nc, l, leaf, [&](int, int a, int k) {
-          const int slot = atomicAdd(&s_ranges, 1);
-          if (slot < n) waves[slot] = make_int2(a, k);
-        });
+        const int nrec = count_records(starts, ends, nc, l, leaf);
+        emit_records(waves, atomicAdd(&s_ranges, nrec), n, starts, ends, nc, l, leaf);
       }
     }
     __syncthreads();
@@ -1102,26 +1159,34 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ pms
 // Wave traversal (the default). A wave takes one range of at most
 // NB_BH_WAVE_BODIES Morton-consecutive bodies of one cell (emitted by the build)
 // and its lanes share every decision: a cell is accepted for the whole range
-// when side^2 < theta^2 dmin^2, dmin = distance from the cell's COM to the
-// range's bounding box; otherwise it is opened, and a leaf that is not
-// accepted is evaluated body by body by every lane. Per body this accepts a
-// subset of what the per-body test accepts (at least as accurate), and nothing
-// diverges. For latency, each wave keeps a stack of opened nodes in LDS as
-// (first child, level of the children << 27 | end of the body range). One
-// iteration pops up to 8 of them and lets 8 lanes per node examine its (at most
-// 8) children at once, so one memory round trip covers 64 nodes. The walk only
-// appends indices to an interaction list in LDS (accepted cells, flagged, and
-// the bodies of unaccepted leaves) and pushes opened children; no source data is
-// read while walking. Whenever the list fills up, and at the end, the wave
-// gathers the sources 64 at a time into LDS and streams them through the
-// direct-sum inner loop (LDS broadcast reads, 13 VALU per interaction).
+// when its opening radius (side / theta + |COM - cell centre|) is less than
+// dmin, the distance from its COM to the range's bounding box; otherwise it is
+// opened, and a leaf that is not accepted is evaluated body by body by every
+// lane. Per body this accepts a subset of what the per-body test accepts (at
+// least as accurate), and nothing diverges. For latency, each wave keeps a
+// stack of opened nodes in LDS as (first child, level of the children << 27 |
+// end of the body range). One iteration pops up to 8 of them and lets 8 lanes
+// per node examine its (at most 8) children at once, so one memory round trip
+// covers 64 nodes. The walk only appends indices to an interaction list in LDS
+// (accepted cells, flagged, and the bodies of unaccepted leaves) and pushes
+// opened children; no source data is read while walking. Whenever the list
+// fills up, and at the end, the wave gathers the sources 64 at a time into LDS
+// and streams them through the direct-sum inner loop (LDS broadcast reads, 13
+// VALU per interaction). The build pairs sibling ranges of equal copy width:
+// such a pair walks once for both ranges and leaves to each range only the
+// cells the two decide differently (walk() below), which on the 4M two-disc
+// system takes 30% fewer walk iterations and stages 34% fewer sources for
+// exactly the same interactions.
 constexpr int kListWaves = 4;
-// LDS (26.7 KB per block) and 74 VGPRs allow six blocks per CU. A 384-entry
+// LDS (26.7 KB per block) and 80 VGPRs allow six blocks per CU. A 384-entry
 // list (instead of 512) is what makes the sixth fit; it measured about 1% faster
 // on the two-disc 4M system.
 constexpr int kListBlocksPerCU = 6;
 // The deepest stack seen on the two-disc 4M system (400 sampled ranges, leaf 8)
-// is 73 entries; overflow is detected and reported (NB_E_TREE_OVERFLOW).
+// is 73 entries. A pair's deferred entries share the region from its top: a CPU
+// replay of the pair walk on 4M-body two-disc, Plummer and uniform-ball systems
+// (leaf 1) needs at most 245 stack plus deferred entries. Overflow is detected
+// and reported (NB_E_TREE_OVERFLOW).
 constexpr int kStackCap = 512;
 constexpr int kListCap = 384;                 // source indices per wave
 constexpr uint32_t kCellBit = 0x80000000u;    // list entry refers to a cell, not a body
@@ -1195,7 +1260,8 @@ __device__ __forceinline__ void list_flush(const uint32_t* list, int lc, float4*
   nbcore::two_sum_fold(sz, cz, az);
 }
 
-// Shared LDS of one traversing wave.
+// Shared LDS of one traversing wave. The stack region holds the walk's stack
+// from the bottom and a pair's deferred entries from the top.
 struct WaveLds {
   int2* stack;
   uint32_t* list;
@@ -1203,36 +1269,50 @@ struct WaveLds {
   const int* lvl;      // first node of each level (kMaxLevel + 2 entries)
 };
 
-// Walk the tree for the bodies of one wave range (lanes past the range hold a
-// copy of a valid body) and accumulate their accelerations.
-template <bool kCount, bool kGuard>
-__device__ void traverse_range(const WaveLds& L, int lane, int P, float4 me, int n,
-                               const float4* __restrict__ pms, const float4* __restrict__ com,
-                               const int4* __restrict__ link, const float* __restrict__ r2,
-                               float eps2, int* __restrict__ flags, float& ax, float& ay, float& az,
-                               float& cx, float& cy, float& cz, unsigned long long& cnt) {
+// An axis-aligned box held in scalar registers (wave-uniform).
+struct Box {
+  float lx, ly, lz, hx, hy, hz;
+};
+
+// Squared distance from c to the box (0 inside).
+__device__ __forceinline__ float box_dist2(const Box& b, const float4& c) {
+  const float ex = fmaxf(fmaxf(b.lx - c.x, c.x - b.hx), 0.f);
+  const float ey = fmaxf(fmaxf(b.ly - c.y, c.y - b.hy), 0.f);
+  const float ez = fmaxf(fmaxf(b.lz - c.z, c.z - b.hz), 0.f);
+  return fmaf(ez, ez, fmaf(ey, ey, ex * ex));
+}
+
+// Inclusive prefix inside each row of 16 lanes (the first four steps of
+// wave_incl_scan).
+__device__ __forceinline__ int row_incl_scan(int v) {
+  v += dpp_shift<0x111, 0xf>(v);
+  v += dpp_shift<0x112, 0xf>(v);
+  v += dpp_shift<0x114, 0xf>(v);
+  v += dpp_shift<0x118, 0xf>(v);
+  return v;
+}
+
+// Walk the tree for the bodies held by the wave's lanes and accumulate their
+// accelerations (lanes of slice s take staged entries s, s + S, ...).
+//   kShared: the shared walk of a pair of ranges with boxes A and B. A cell
+//     both accept is evaluated for both ranges; a cell exactly one accepts is
+//     deferred: its (first sibling, 8-bit sibling mask) entry goes to the top of
+//     the stack region and nd counts them. Cells neither accepts are opened (or,
+//     as leaves, evaluated body by body) for both.
+//   otherwise: the walk of one range with box U, from the root when nd == 0,
+//     else from the nd deferred entries (read, not consumed: the pair's other
+//     range walks them too).
+// For each range this examines and evaluates exactly the cells its own walk
+// from the root would: the shared walk only descends where both ranges open,
+// and a deferred cell is decided by each range's own test.
+template <bool kCount, bool kGuard, bool kShared>
+__device__ void walk(const WaveLds& L, int lane, int slice, int log2S, float4 me, int n,
+                     const float4* __restrict__ pms, const float4* __restrict__ com,
+                     const int4* __restrict__ link, const float* __restrict__ r2, float eps2,
+                     int* __restrict__ flags, const Box& U, const Box& A, const Box& B, int& nd,
+                     float& ax, float& ay, float& az, float& cx, float& cy, float& cz,
+                     unsigned long long& cnt) {
   const uint64_t lt = (uint64_t(1) << lane) - 1;
-  const int log2S = 6 - (31 - __clz(P)), slice = lane / P;  // S = 64 / P
-  float lx = me.x, ly = me.y, lz = me.z, hx = me.x, hy = me.y, hz = me.z;
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    lx = fminf(lx, __shfl_xor(lx, off));
-    ly = fminf(ly, __shfl_xor(ly, off));
-    lz = fminf(lz, __shfl_xor(lz, off));
-    hx = fmaxf(hx, __shfl_xor(hx, off));
-    hy = fmaxf(hy, __shfl_xor(hy, off));
-    hz = fmaxf(hz, __shfl_xor(hz, off));
-  }
-  // the box is the same in every lane: keep it in scalar registers
-  auto uniform = [](float v) {
-    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
-  };
-  lx = uniform(lx);
-  ly = uniform(ly);
-  lz = uniform(lz);
-  hx = uniform(hx);
-  hy = uniform(hy);
-  hz = uniform(hz);
   int2* stack = L.stack;
   uint32_t* list = L.list;
 #define NB_FLUSH()                                                                              \
@@ -1241,64 +1321,115 @@ __device__ void traverse_range(const WaveLds& L, int lane, int P, float4 me, int
                                az, cx, cy, cz, cnt);                                           \
     lc = 0;                                                                                     \
   } while (0)
-  // a virtual parent whose only child is the root
-  if (lane == 0) stack[0] = make_int2(0, (int)((1u << 27) | (uint32_t)n));
-  __builtin_amdgcn_wave_barrier();
-  int sp = 1, lc = 0;
-  for (int iter = 0; sp > 0; ++iter) {
+  int sp = 0, lc = 0, dpos = 0;
+  if (kShared || nd == 0) {
+    // a virtual parent whose only child is the root
+    if (lane == 0) stack[0] = make_int2(0, (int)((1u << 27) | (uint32_t)n));
+    sp = 1;
+    __builtin_amdgcn_wave_barrier();
+  }
+  const int q = lane >> 3, ci = lane & 7;
+  for (int iter = 0; sp > 0 || (!kShared && dpos < nd); ++iter) {
     if (iter > L.lvl[kMaxLevel + 1]) {  // every node is examined at most once
       if (lane == 0) atomicOr(flags, 4);
       break;
     }
-    const int k = min(sp, 8);
-    sp -= k;
-    const int q = lane >> 3, ci = lane & 7;
-    int child = -1;
+    // room for this iteration's accepted cells (at most 64), flushed while
+    // nothing of the iteration is live
+    if (lc + 64 > kListCap) NB_FLUSH();
+    int child = -1, efirst = 0;
     int4 cl = make_int4(-1, -1, 0, 0);
     float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
     float rr = 0.f;
-    if (q < k) {
-      // entry: first child, and (slots << 27 | parent's body end); the slots stop
-      // at the end of the children's level, so every load below is in bounds and
-      // the three are issued together (a child is one whose bodies start before
-      // the parent's end)
-      const int2 e = stack[sp + q];
-      const uint32_t w = (uint32_t)e.y;
-      const int slots = (int)(w >> 27);
-      const int pend = (int)(w & ((1u << 27) - 1));
-      if (ci < slots) {
-        const int cand = e.x + ci;
-        cl = link[cand];
-        c = com[cand];
-        rr = r2[cand];
-        if (cl.z < pend) child = cand;
+    if (!kShared && sp < 8 && dpos < nd) {
+      // deferred entries: up to 16 of them, at most 64 cells, listed in the
+      // staging area (free while walking) so that lane j takes the j-th cell
+      int2 d = make_int2(0, 0);
+      const bool have = lane < 16 && dpos + lane < nd;
+      if (have) d = stack[kStackCap - 1 - (dpos + lane)];
+      const int pc = __popc((uint32_t)d.y);
+      const int incl = row_incl_scan(pc);
+      const uint64_t fit = __ballot(have && incl <= 64);
+      const int kd = __popcll(fit);  // >= 1: one entry has at most 8 cells
+      int* cells = reinterpret_cast<int*>(L.stage);
+      if ((fit >> lane) & 1) {
+        int o = incl - pc;
+        for (uint32_t mm = (uint32_t)d.y; mm; mm &= mm - 1) cells[o++] = d.x + __ffs(mm) - 1;
       }
+      __builtin_amdgcn_wave_barrier();
+      const int T = __builtin_amdgcn_readlane(incl, kd - 1);
+      if (lane < T) {
+        child = cells[lane];
+        cl = link[child];
+        c = com[child];
+        rr = r2[child];
+      }
+      __builtin_amdgcn_wave_barrier();
+      dpos += kd;
+    } else {
+      const int k = min(sp, 8);
+      sp -= k;
+      if (q < k) {
+        // entry: first child, and (slots << 27 | parent's body end); the slots
+        // stop at the end of the children's level, so every load below is in
+        // bounds and the three are issued together (a child is one whose bodies
+        // start before the parent's end)
+        const int2 e = stack[sp + q];
+        efirst = e.x;
+        const uint32_t w = (uint32_t)e.y;
+        const int slots = (int)(w >> 27);
+        const int pend = (int)(w & ((1u << 27) - 1));
+        if (ci < slots) {
+          const int cand = e.x + ci;
+          cl = link[cand];
+          c = com[cand];
+          rr = r2[cand];
+          if (cl.z < pend) child = cand;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
     }
-    __builtin_amdgcn_wave_barrier();
-    bool accept = false, leaf = false, open = false;
+    bool accept = false, defer = false, leaf = false, open = false;
     if (child >= 0) {
-      const float ex = fmaxf(fmaxf(lx - c.x, c.x - hx), 0.f);
-      const float ey = fmaxf(fmaxf(ly - c.y, c.y - hy), 0.f);
-      const float ez = fmaxf(fmaxf(lz - c.z, c.z - hz), 0.f);
-      const float dmin2 = fmaf(ez, ez, fmaf(ey, ey, ex * ex));
-      accept = rr < dmin2;  // the group's modified test (see open_radius2)
-      leaf = !accept && cl.x < 0;
-      open = !accept && cl.x >= 0;
+      // the group's modified test (see open_radius2); a pair shares what both accept
+      if (kShared) {
+        const bool a = rr < box_dist2(A, c), b = rr < box_dist2(B, c);
+        accept = a && b;
+        defer = a != b;
+      } else {
+        accept = rr < box_dist2(U, c);
+      }
+      leaf = !accept && !defer && cl.x < 0;
+      open = !accept && !defer && cl.x >= 0;
     }
     // accepted cells with mass -> list
     const bool acc = accept && c.w > 0.f;
     const uint64_t ba = __ballot(acc);
-    if (ba) {
-      if (lc + 64 > kListCap) NB_FLUSH();
-      if (acc) list[lc + __popcll(ba & lt)] = (uint32_t)child | kCellBit;
-      lc += __popcll(ba);
+    if (acc) list[lc + __popcll(ba & lt)] = (uint32_t)child | kCellBit;
+    lc += __popcll(ba);
+    // deferred cells -> one entry per stack entry that has any
+    int nd_add = 0;
+    bool head = false;
+    uint32_t dmask = 0;
+    uint64_t bg = 0;
+    if (kShared) {
+      const uint64_t bd = __ballot(defer);
+      if (bd) {
+        const uint32_t half = q < 4 ? (uint32_t)bd : (uint32_t)(bd >> 32);
+        dmask = (half >> (lane & 24)) & 0xffu;  // bits 8q .. 8q + 7
+        head = ci == 0 && dmask != 0;
+        bg = __ballot(head);
+        nd_add = __popcll(bg);
+      }
     }
     // opened cells -> stack
     const uint64_t bo = __ballot(open);
-    if (sp + __popcll(bo) > kStackCap) {
+    if (sp + __popcll(bo) + nd + nd_add > kStackCap) {
       if (lane == 0) atomicOr(flags, 8);
       break;
     }
+    if (kShared && head) stack[kStackCap - 1 - (nd + __popcll(bg & lt))] = make_int2(efirst, (int)dmask);
+    nd += nd_add;
     if (open) {
       const uint32_t pend = (uint32_t)(cl.z + link_count(cl.w));
       const int slots = max(0, min(8, L.lvl[link_level(cl.w) + 2] - cl.x));
@@ -1325,10 +1456,51 @@ __device__ void traverse_range(const WaveLds& L, int lane, int P, float4 me, int
 #undef NB_FLUSH
 }
 
-// Persistent: every wave takes wave ranges from a ticket until none is left,
-// so the cost of uneven ranges balances dynamically. The counting variant
-// (diagnostics only) needs more registers and is allowed five blocks per CU;
-// its surplus blocks start when others retire and take what tickets are left.
+// The bounding box of the bodies held by the lanes, reduced over every lane bit
+// except `keep` (0: over all lanes; else each half of the lanes split by that
+// bit gets its own box), then read from lanes `from0` and `from1`.
+__device__ __forceinline__ void lane_boxes(float4 me, int keep, int from0, int from1, Box& b0,
+                                           Box& b1) {
+  float lx = me.x, ly = me.y, lz = me.z, hx = me.x, hy = me.y, hz = me.z;
+  // lanes lane ^ off: ds_swizzle's bit mode inside each half of the wave, one
+  // ds_bpermute across the halves
+  auto step = [&](auto xor_lanes) {
+    lx = fminf(lx, xor_lanes(lx));
+    ly = fminf(ly, xor_lanes(ly));
+    lz = fminf(lz, xor_lanes(lz));
+    hx = fmaxf(hx, xor_lanes(hx));
+    hy = fmaxf(hy, xor_lanes(hy));
+    hz = fmaxf(hz, xor_lanes(hz));
+  };
+  auto swz = [](auto off_c) {
+    return [](float v) {
+      constexpr int kPattern = (decltype(off_c)::value << 10) | 0x1f;  // xor, or 0, and 0x1f
+      return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), kPattern));
+    };
+  };
+  if (keep != 32) step([](float v) { return __shfl_xor(v, 32); });
+  if (keep != 16) step(swz(std::integral_constant<int, 16>{}));
+  if (keep != 8) step(swz(std::integral_constant<int, 8>{}));
+  if (keep != 4) step(swz(std::integral_constant<int, 4>{}));
+  if (keep != 2) step(swz(std::integral_constant<int, 2>{}));
+  if (keep != 1) step(swz(std::integral_constant<int, 1>{}));
+  // uniform over each group: keep the boxes in scalar registers
+  auto at = [](float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+  };
+  b0 = Box{at(lx, from0), at(ly, from0), at(lz, from0), at(hx, from0), at(hy, from0), at(hz, from0)};
+  b1 = Box{at(lx, from1), at(ly, from1), at(lz, from1), at(hx, from1), at(hy, from1), at(hz, from1)};
+}
+
+// Persistent: every wave takes records from a ticket until none is left, so the
+// cost of uneven ranges balances dynamically. A record is one wave range or a
+// pair of sibling ranges (x, y) and (z, w), w > 0. A pair first walks once for
+// both (its lanes: [slice][range][slot], P1 lanes per copy of a range), then,
+// if cells were deferred, each range walks them with its own lanes ([slice][slot]);
+// each body's force is the sum of the two parts in FP64, in that order. The
+// counting variant (diagnostics only) needs more registers and is allowed five
+// blocks per CU; its surplus blocks start when others retire and take what
+// tickets are left.
 template <bool kCount, bool kGuard>
 __global__ __launch_bounds__(64 * kListWaves, kCount ? 5 : kListBlocksPerCU) void k_traverse_list(
     const float4* __restrict__ pms, const uint32_t* __restrict__ perm, int n,
@@ -1336,7 +1508,8 @@ __global__ __launch_bounds__(64 * kListWaves, kCount ? 5 : kListBlocksPerCU) voi
     const int* __restrict__ level_off, int64_t max_nodes, const float* __restrict__ r2,
     float eps2, double G, double* __restrict__ force,
     unsigned long long* __restrict__ inter, int* __restrict__ flags,
-    const int2* __restrict__ waves, int* __restrict__ wctl) {
+    const int4* __restrict__ waves, int* __restrict__ wctl, const int* __restrict__ order,
+    float* __restrict__ cost) {
   __shared__ int2 s_stack[kListWaves][kStackCap];
   __shared__ uint32_t s_list[kListWaves][kListCap];
   __shared__ float4 s_stage[kListWaves][64];
@@ -1347,41 +1520,94 @@ __global__ __launch_bounds__(64 * kListWaves, kCount ? 5 : kListBlocksPerCU) voi
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const WaveLds L{s_stack[wave], s_list[wave], s_stage[wave], s_lvl};
-  const int nranges = min(wctl[0], n);
+  const int nrec = min(wctl[0], n);
   unsigned long long cnt = 0;
   for (;;) {
     int w = 0;
     if (lane == 0) w = atomicAdd(&wctl[1], 1);
-    w = __shfl(w, 0);
-    if (w >= nranges) break;
-    const int2 range = waves[w];
-    // P lanes per copy of the range's bodies: a power of two >= its count, >= 4
-    const int P = range.y <= 4 ? 4 : 1 << (32 - __clz(range.y - 1));
-    const int slot = lane & (P - 1);
-    const bool valid = slot < range.y;
-    const int t = min(range.x + (valid ? slot : 0), n - 1);
+    w = __builtin_amdgcn_readfirstlane(w);
+    if (w >= nrec) break;
+    const uint64_t t_start = wall_clock64();
+    const int4 rec = waves[order[w]];  // wave-uniform: scalar registers
+    const bool pair = rec.w > 0;
+    // shared walk: a single range is held 64 / P1 times, a pair 32 / P1 times
+    const int P1 = copy_lanes(pair ? max(rec.y, rec.w) : rec.y);
+    const int grp = pair ? 2 * P1 : P1;
+    const int slot = lane & (P1 - 1);
+    const bool second = pair && (lane & P1);
+    const int rs = second ? rec.z : rec.x, rn = second ? rec.w : rec.y;
+    const bool valid = slot < rn;
+    const int t = min(rs + (valid ? slot : 0), n - 1);  // lanes past a range copy its first body
     const float4 me = pms[t];
+    const int log2S = 6 - (31 - __clz(grp)), slice = lane / grp;
+    Box A, B;
+    lane_boxes(me, pair ? P1 : 0, 0, pair ? P1 : 0, A, B);
     float ax = 0.f, ay = 0.f, az = 0.f, cx = 0.f, cy = 0.f, cz = 0.f;
     unsigned long long c = 0;
-    traverse_range<kCount, kGuard>(L, lane, P, me, n, pms, com, link, r2, eps2, flags, ax,
-                                   ay, az, cx, cy, cz, c);
+    int nd = 0;
+    if (pair)
+      walk<kCount, kGuard, true>(L, lane, slice, log2S, me, n, pms, com, link, r2, eps2, flags, A,
+                                 A, B, nd, ax, ay, az, cx, cy, cz, c);
+    else
+      walk<kCount, kGuard, false>(L, lane, slice, log2S, me, n, pms, com, link, r2, eps2, flags,
+                                  A, A, B, nd, ax, ay, az, cx, cy, cz, c);
     // each slice's compensated sum in FP64, then the slices added (butterfly
     // over the slice bits, fixed order)
     double fx = (double)ax + (double)cx, fy = (double)ay + (double)cy,
            fz = (double)az + (double)cz;
-    for (int off = P; off < 64; off <<= 1) {
+    for (int off = grp; off < 64; off <<= 1) {
       fx += __shfl_xor(fx, off);
       fy += __shfl_xor(fy, off);
       fz += __shfl_xor(fz, off);
     }
     if (valid) cnt += c;
-    if (valid && lane < P) {
-      const uint32_t o = perm[t];
-      if (o < (uint32_t)n) {
-        force[o] = G * fx;
-        force[n + o] = G * fy;
-        force[2 * n + o] = G * fz;
+    if (nd == 0) {
+      if (valid && lane < grp) {
+        const uint32_t o = perm[t];
+        if (o < (uint32_t)n) {
+          force[o] = G * fx;
+          force[n + o] = G * fy;
+          force[2 * n + o] = G * fz;
+        }
       }
+    }
+    // the deferred cells, range by range
+    for (int h = 0; h < (nd > 0 ? 2 : 0); ++h) {
+      const int hs = h ? rec.z : rec.x, hn = h ? rec.w : rec.y;
+      const int P = copy_lanes(hn);
+      const int s2 = lane & (P - 1);
+      const bool v2 = s2 < hn;
+      const int t2 = min(hs + (v2 ? s2 : 0), n - 1);
+      const float4 me2 = pms[t2];
+      ax = ay = az = cx = cy = cz = 0.f;
+      c = 0;
+      walk<kCount, kGuard, false>(L, lane, lane / P, 6 - (31 - __clz(P)), me2, n, pms, com, link,
+                                  r2, eps2, flags, h ? B : A, A, B, nd, ax, ay, az, cx, cy, cz, c);
+      double gx = (double)ax + (double)cx, gy = (double)ay + (double)cy,
+             gz = (double)az + (double)cz;
+      for (int off = P; off < 64; off <<= 1) {
+        gx += __shfl_xor(gx, off);
+        gy += __shfl_xor(gy, off);
+        gz += __shfl_xor(gz, off);
+      }
+      // the shared part of body s2 of range h sits in lane h * P1 + s2
+      const double px = __shfl(fx, h * P1 + s2), py = __shfl(fy, h * P1 + s2),
+                   pz = __shfl(fz, h * P1 + s2);
+      if (v2) cnt += c;
+      if (v2 && lane < P) {
+        const uint32_t o = perm[t2];
+        if (o < (uint32_t)n) {
+          force[o] = G * (px + gx);
+          force[n + o] = G * (py + gy);
+          force[2 * n + o] = G * (pz + gz);
+        }
+      }
+    }
+    // the record's device time, shared out over its bodies for the next pass's order
+    const float share = (float)(wall_clock64() - t_start) / (float)(rec.y + rec.w);
+    if (valid && lane < grp) {
+      const uint32_t o = perm[t];
+      if (o < (uint32_t)n) cost[o] = share;
     }
   }
   if (kCount) {
@@ -1393,6 +1619,85 @@ __global__ __launch_bounds__(64 * kListWaves, kCount ? 5 : kListBlocksPerCU) voi
       __syncthreads();
     }
     if (threadIdx.x == 0) atomicAdd(inter, red[0]);
+  }
+}
+
+// ---- 8. traversal order ---------------------------------------------------------------
+// A persistent launch ends when its last wave finishes its last record, so the
+// records are taken costliest first: each record's estimate is the device time
+// its bodies' records took in the previous pass (stored per body, by original
+// index, so it survives the rebuild), bucketed by half-octaves. Within a bucket
+// the order is that of the atomics. The order changes only which wave takes a
+// record and when; every record's arithmetic, and so every force, is the same.
+
+// Half-octave of an estimate; anything not a finite value >= 1 (no history yet,
+// or memory not written by a pass) goes last.
+__device__ __forceinline__ int cost_bucket(float est) {
+  if (!(est >= 1.f) || !(est <= FLT_MAX)) return 0;
+  const uint32_t u = __float_as_uint(est);
+  return min(kCostBuckets - 1, 2 * ((int)(u >> 23) - 127) + (int)((u >> 22) & 1));
+}
+
+// Records are handled in chunks of kOrderChunk by a grid of at most kOrderBlocks
+// blocks; each block counts in LDS and makes one global atomic per bucket per
+// chunk, so the few busy buckets see few atomics.
+constexpr int kOrderChunk = 1024;
+constexpr int kOrderBlocks = 256;
+
+__global__ __launch_bounds__(256) void k_rec_bucket(const int4* __restrict__ waves,
+                                                    const int* __restrict__ wctl, int n,
+                                                    const float* __restrict__ cost,
+                                                    const uint32_t* __restrict__ perm,
+                                                    int* __restrict__ bucket,
+                                                    int* __restrict__ hist) {
+  __shared__ int h[kCostBuckets];
+  if (threadIdx.x < kCostBuckets) h[threadIdx.x] = 0;
+  __syncthreads();
+  const int nrec = min(wctl[0], n);
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < nrec; i += gridDim.x * 256) {
+    const int4 r = waves[i];  // starts are sorted body indices < n
+    float est = cost[perm[r.x]] * (float)r.y;
+    if (r.w > 0) est += cost[perm[r.z]] * (float)r.w;
+    const int b = cost_bucket(est);
+    bucket[i] = b;
+    atomicAdd(&h[b], 1);
+  }
+  __syncthreads();
+  if (threadIdx.x < kCostBuckets && h[threadIdx.x] > 0) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// One wave: cursor[b] = records in buckets above b (costliest first).
+__global__ __launch_bounds__(64) void k_rec_offsets(int* __restrict__ hist) {
+  const int lane = threadIdx.x;
+  const int v = hist[kCostBuckets - 1 - lane];
+  hist[kCostBuckets + kCostBuckets - 1 - lane] = wave_incl_scan(v) - v;
+}
+
+__global__ __launch_bounds__(256) void k_rec_scatter(const int* __restrict__ wctl, int n,
+                                                     const int* __restrict__ bucket,
+                                                     int* __restrict__ hist,
+                                                     int* __restrict__ order) {
+  __shared__ int h[kCostBuckets], base[kCostBuckets];
+  constexpr int kPer = kOrderChunk / 256;
+  const int nrec = min(wctl[0], n);
+  for (int c0 = blockIdx.x * kOrderChunk; c0 < nrec; c0 += gridDim.x * kOrderChunk) {
+    if (threadIdx.x < kCostBuckets) h[threadIdx.x] = 0;
+    __syncthreads();
+    int bk[kPer], pos[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+      const int i = c0 + k * 256 + threadIdx.x;
+      bk[k] = i < nrec ? bucket[i] : -1;
+      pos[k] = bk[k] >= 0 ? atomicAdd(&h[bk[k]], 1) : 0;
+    }
+    __syncthreads();
+    if (threadIdx.x < kCostBuckets && h[threadIdx.x] > 0)
+      base[threadIdx.x] = atomicAdd(&hist[kCostBuckets + threadIdx.x], h[threadIdx.x]);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+      if (bk[k] >= 0) order[base[bk[k]] + pos[k]] = c0 + k * 256 + threadIdx.x;
+    __syncthreads();
   }
 }
 
@@ -1450,8 +1755,12 @@ void nb_bh_carve(nb_ctx* c, char* base, int upto) {
   b.child_bounds = (int*)at(kBhChildBounds);
   b.counters = (int*)at(kBhCounters);
   b.top_start = (int*)at(kBhTopStart);
-  b.waves = (int2*)at(kBhWaves);
+  b.waves = (int4*)at(kBhWaves);
   b.inter = (unsigned long long*)at(kBhInter);
+  b.rec_cost = (float*)at(kBhRecCost);
+  b.rec_bucket = (int*)at(kBhRecBucket);
+  b.rec_order = (int*)at(kBhRecOrder);
+  b.rec_hist = (int*)at(kBhRecHist);
   b.bbox_blocks = kBboxBlocks;
   b.sort_blocks = (int)L.sort_blocks;
   b.max_nodes = upto > kBhPms ? L.max_nodes : 0;
@@ -1488,7 +1797,8 @@ int nb_bh_alloc(nb_ctx* c) {
   nb_bh_carve(c, (char*)b.block, kBhBufCount);
   // the batched leaf loop may read up to kSpare bodies past the end: zero mass
   if (hipMemset(b.pms + c->n, 0, sizeof(float4) * kSpare) != hipSuccess ||
-      hipMemset(b.geom, 0, sizeof(float) * 32) != hipSuccess)
+      hipMemset(b.geom, 0, sizeof(float) * 32) != hipSuccess ||
+      hipMemset(b.rec_cost, 0, sizeof(float) * c->n) != hipSuccess)
     return nb_fail(NB_E_HIP, "nb_create: hipMemset failed (octree)");
   return NB_OK;
   NB_CATCH("nb_bh_alloc")
@@ -1568,7 +1878,7 @@ hipError_t nb_bh_build(nb_ctx* c) {
   hipLaunchKernelGGL(k_top_bounds, dim3((kTopEntries + 255) / 256), dim3(256), 0, s, codes_sorted,
                      n, b.top_start);
   hipLaunchKernelGGL(k_top_build, dim3(1), dim3(1024), 0, s, b.top_start, n, leaf, b.node_link,
-                     b.level_off, b.waves, b.counters);
+                     b.level_off, b.waves, b.counters, b.rec_hist);
   for (int l = kTopLevels; l < kMaxLevel; ++l) {
     const int bound = level_bound(n, l);
     const int grid = std::max(1, std::min((bound + 255) / 256, 4096));
@@ -1602,9 +1912,18 @@ hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions) {
       c->cfg.eps2, c->G, c->force, b.inter, c->flags
   const bool guard = c->guard;
   const int gl = b.persist_blocks;  // persistent blocks pulling wave ranges
+  if (group) {
+    const int og = std::max(1, std::min((n + kOrderChunk - 1) / kOrderChunk, kOrderBlocks));
+    hipLaunchKernelGGL(k_rec_bucket, dim3(og), dim3(256), 0, c->stream, b.waves, b.counters, n,
+                       b.rec_cost, b.vals[0], b.rec_bucket, b.rec_hist);
+    hipLaunchKernelGGL(k_rec_offsets, dim3(1), dim3(64), 0, c->stream, b.rec_hist);
+    hipLaunchKernelGGL(k_rec_scatter, dim3(og), dim3(256), 0, c->stream, b.counters, n,
+                       b.rec_bucket, b.rec_hist, b.rec_order);
+  }
 #define NB_LIST_ARGS                                                                     \
   b.pms, b.vals[0], n, b.node_com, b.node_link, b.level_off, b.max_nodes, b.node_r2,     \
-      c->cfg.eps2, c->G, c->force, b.inter, c->flags, b.waves, b.counters
+      c->cfg.eps2, c->G, c->force, b.inter, c->flags, b.waves, b.counters, b.rec_order,  \
+      b.rec_cost
   if (group && count_interactions && guard)
     hipLaunchKernelGGL((k_traverse_list<true, true>), dim3(gl), dim3(64 * kListWaves), 0, c->stream,
                        NB_LIST_ARGS);

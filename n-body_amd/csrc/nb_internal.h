@@ -82,12 +82,13 @@ constexpr int kSpare = 4;                             // zero-mass bodies after 
 constexpr int kTopLevels = 4;                         // levels built from per-cell searches
 constexpr int kTopCells = 1 + 8 + 64 + 512 + 4096;    // cells of levels 0..kTopLevels
 constexpr int kTopEntries = kTopCells + kTopLevels + 1;  // plus an end sentinel per level
+constexpr int kCostBuckets = 64;  // half-octaves of estimated record time
 
 enum nb_bh_buf {
   kBhBbox, kBhGeom, kBhKey0, kBhKey1, kBhVal0, kBhVal1, kBhCodes, kBhHist, kBhIscan,
   kBhPms, kBhPrefix, kBhScanPart, kBhCom, kBhR2, kBhLink, kBhLevel, kBhChildCnt,
   kBhChildOff, kBhChildBounds, kBhCounters, kBhTopStart, kBhWaves, kBhInter, kBhFlags,
-  kBhBufCount
+  kBhRecCost, kBhRecBucket, kBhRecOrder, kBhRecHist, kBhBufCount
 };
 
 // Node bound: a level holds at most min(8^l, n) nodes and there are 11 levels.
@@ -128,9 +129,11 @@ constexpr nb_bh_layout nb_bh_layout_for(int64_t n) {
   bytes[kBhChildBounds] = 4 * 8 * n;
   bytes[kBhCounters] = 4 * 16;
   bytes[kBhTopStart] = 4 * kTopEntries;
-  bytes[kBhWaves] = 8 * n;
+  bytes[kBhWaves] = 16 * n;
   bytes[kBhInter] = 8 * 2;
   bytes[kBhFlags] = 4 * 4;
+  bytes[kBhRecCost] = bytes[kBhRecBucket] = bytes[kBhRecOrder] = 4 * n;
+  bytes[kBhRecHist] = 4 * 2 * kCostBuckets;
   int64_t at = 0;
   for (int k = 0; k < kBhBufCount; ++k) {
     L.off[k] = at;
@@ -173,8 +176,13 @@ struct nb_bh_state {
   int* child_bounds = nullptr;  // per node of the current level: its 7 inner child boundaries [8 n]
   int* iscan_part = nullptr;    // int scan block partials
   int* counters = nullptr;      // [0] wave ranges, [1] traversal ticket
-  int2* waves = nullptr;        // wave ranges (first sorted body, count <= 64)
+  int4* waves = nullptr;        // records of one or two sibling wave ranges (start, count, start, count)
   int* top_start = nullptr;     // first sorted body of every cell of the top levels
+  // traversal order: records by the device time their bodies took in the last pass
+  float* rec_cost = nullptr;    // per body (original index): last pass's time of its record / its bodies
+  int* rec_bucket = nullptr;    // per record: cost bucket of this pass
+  int* rec_order = nullptr;     // tickets -> records, costliest buckets first
+  int* rec_hist = nullptr;      // [kCostBuckets] records per bucket, then [kCostBuckets] cursors
   int persist_blocks = 0;       // traversal workgroups (persistent)
   unsigned long long* inter = nullptr;  // interaction counter (diagnostic traversal)
 };

@@ -147,6 +147,26 @@ def test_root_leaf_with_several_ranges(nb, ora, n, leaf):
     assert rel_err(f, ref).max() < 1e-5
 
 
+def test_deep_leaf_of_many_ranges(nb, ora):
+    # 9000 coincident bodies make one level-10 leaf cut into 282 ranges (281 of
+    # 32 bodies and one of 8): far more ranges of one width than a node's
+    # children give, paired two by two; the rest is a ball around them
+    n_pt, n_ball = 9000, 3000
+    bp, _, bm = nb.ic_ball(77, n_ball, 50.0, 1.0)
+    pos = np.concatenate([np.tile(np.array([[3.0, -2.0, 1.0]], np.float32), (n_pt, 1)), bp])
+    m = np.concatenate([np.full(n_pt, 0.5, np.float32), bm])
+    n = len(m)
+    f, codes, perm, nodes, bbox, inter = gpu_run(nb, pos, np.zeros((n, 3)), m, 4)
+    assert (nodes["body_count"][nodes["level"] == 10] >= n_pt).any()
+    p4 = pm4(pos, m)
+    mn, mx, ccodes, cperm, tree, com = cpu_build(ora, p4, 4)
+    np.testing.assert_array_equal(perm, cperm)
+    fc, cinter = ora.bh_forces_group(p4[cperm], cperm, mn, mx, tree, com, EPS2,
+                                     group=nb.BH_WAVE_BODIES, codes_sorted=ccodes[cperm])
+    assert inter == cinter
+    assert np.percentile(rel_err(f, fc), 99) < 1e-5
+
+
 def test_all_bodies_coincident(nb):
     n = 100
     pos = np.tile(np.array([[1.0, 2.0, 3.0]]), (n, 1))
@@ -184,6 +204,23 @@ def test_determinism_and_graph(nb):
             outs.append(sim.download_f64())
     for x, y in zip(*outs):
         assert np.array_equal(x, y)
+
+
+def test_forces_independent_of_traversal_order(nb):
+    # the traversal takes its work items costliest first by the device time
+    # they took in the previous pass: a pass with no history, one after a
+    # pass of the same state, and one after passes of other states must agree
+    # bit for bit
+    n = 200000
+    pos, vel, m = nb.ic_discs(9, n)
+    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH, leaf_size=1) as sim:
+        sim.upload(pos, vel, m)
+        f0 = sim.forces_f64()
+        f1 = sim.forces_f64()
+        sim.step(3)
+        sim.upload(pos, vel, m)
+        f2 = sim.forces_f64()
+    assert np.array_equal(f0, f1) and np.array_equal(f0, f2)
 
 
 def test_c3_discs_properties(nb, ora):
