@@ -10,7 +10,7 @@
 
 constexpr int64_t kScratchBytes = nb_bh_layout_for(kNoAllocMaxBodies).off[kBhBufCount];
 
-// The scratch of every call on a device (20.9 MB for 65536 bodies). In host
+// The scratch of every call on a device (22.2 MB for 65536 bodies). In host
 // code this name is only a handle for hipGetSymbolAddress: the device address of
 // each GPU's copy is looked up once per device and kept in g_scratch.
 static __device__ __align__(256) unsigned char scratch[kScratchBytes];
