@@ -211,6 +211,12 @@ typedef struct nb_node_view {
 int nb_debug_tree(nb_ctx* ctx, nb_node_view* nodes, int64_t cap, int64_t* node_count,
                   float* bbox);
 
+/* Cap the default traversal's per-wave LDS stack region (its stack of opened
+ * nodes plus a pair's deferred entries) at `entries`, 1..512; 0 restores 512.
+ * For tests of the overflow path: a walk that would exceed the cap stops, and
+ * the pass returns NB_E_TREE_OVERFLOW. No counterpart in the reference. */
+int nb_debug_set_walk_stack(nb_ctx* ctx, int32_t entries);
+
 /* ---- timing ------------------------------------------------------------ */
 typedef struct nb_timing {
   double force_ms;      /* summed device time of force kernels (direct or traversal) */

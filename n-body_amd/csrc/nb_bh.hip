@@ -1267,6 +1267,7 @@ struct WaveLds {
   uint32_t* list;
   float4* stage;
   const int* lvl;      // first node of each level (kMaxLevel + 2 entries)
+  int cap;             // entries of the stack region in use (<= kStackCap)
 };
 
 // An axis-aligned box held in scalar registers (wave-uniform).
@@ -1346,7 +1347,7 @@ __device__ void walk(const WaveLds& L, int lane, int slice, int log2S, float4 me
       // staging area (free while walking) so that lane j takes the j-th cell
       int2 d = make_int2(0, 0);
       const bool have = lane < 16 && dpos + lane < nd;
-      if (have) d = stack[kStackCap - 1 - (dpos + lane)];
+      if (have) d = stack[L.cap - 1 - (dpos + lane)];
       const int pc = __popc((uint32_t)d.y);
       const int incl = row_incl_scan(pc);
       const uint64_t fit = __ballot(have && incl <= 64);
@@ -1424,11 +1425,11 @@ __device__ void walk(const WaveLds& L, int lane, int slice, int log2S, float4 me
     }
     // opened cells -> stack
     const uint64_t bo = __ballot(open);
-    if (sp + __popcll(bo) + nd + nd_add > kStackCap) {
+    if (sp + __popcll(bo) + nd + nd_add > L.cap) {
       if (lane == 0) atomicOr(flags, 8);
       break;
     }
-    if (kShared && head) stack[kStackCap - 1 - (nd + __popcll(bg & lt))] = make_int2(efirst, (int)dmask);
+    if (kShared && head) stack[L.cap - 1 - (nd + __popcll(bg & lt))] = make_int2(efirst, (int)dmask);
     nd += nd_add;
     if (open) {
       const uint32_t pend = (uint32_t)(cl.z + link_count(cl.w));
@@ -1509,7 +1510,7 @@ __global__ __launch_bounds__(64 * kListWaves, kCount ? 5 : kListBlocksPerCU) voi
     float eps2, double G, double* __restrict__ force,
     unsigned long long* __restrict__ inter, int* __restrict__ flags,
     const int4* __restrict__ waves, int* __restrict__ wctl, const int* __restrict__ order,
-    float* __restrict__ cost) {
+    float* __restrict__ cost, int stack_cap) {
   __shared__ int2 s_stack[kListWaves][kStackCap];
   __shared__ uint32_t s_list[kListWaves][kListCap];
   __shared__ float4 s_stage[kListWaves][64];
@@ -1519,7 +1520,8 @@ __global__ __launch_bounds__(64 * kListWaves, kCount ? 5 : kListBlocksPerCU) voi
   __syncthreads();
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const WaveLds L{s_stack[wave], s_list[wave], s_stage[wave], s_lvl};
+  const WaveLds L{s_stack[wave], s_list[wave], s_stage[wave], s_lvl,
+                  stack_cap > 0 && stack_cap < kStackCap ? stack_cap : kStackCap};
   const int nrec = min(wctl[0], n);
   unsigned long long cnt = 0;
   for (;;) {
@@ -1923,7 +1925,7 @@ hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions) {
 #define NB_LIST_ARGS                                                                     \
   b.pms, b.vals[0], n, b.node_com, b.node_link, b.level_off, b.max_nodes, b.node_r2,     \
       c->cfg.eps2, c->G, c->force, b.inter, c->flags, b.waves, b.counters, b.rec_order,  \
-      b.rec_cost
+      b.rec_cost, c->walk_stack
   if (group && count_interactions && guard)
     hipLaunchKernelGGL((k_traverse_list<true, true>), dim3(gl), dim3(64 * kListWaves), 0, c->stream,
                        NB_LIST_ARGS);
@@ -1956,6 +1958,18 @@ hipError_t nb_bh_count_per_body(nb_ctx* c) {
 }
 
 // ---- debug taps ---------------------------------------------------------------------------------
+extern "C" int nb_debug_set_walk_stack(nb_ctx* c, int32_t entries) {
+  NB_TRY
+  if (!c) return nb_fail(NB_E_ARG, "nb_debug_set_walk_stack: null context");
+  if (entries < 0 || entries > kStackCap)
+    return nb_fail(NB_E_ARG, "nb_debug_set_walk_stack: entries must be 0..512");
+  // a captured step holds the old value as a kernel argument
+  nb_drop_graph(c->stream, c->graph_exec, c->graph);
+  c->walk_stack = entries;
+  return NB_OK;
+  NB_CATCH("nb_debug_set_walk_stack")
+}
+
 extern "C" int nb_debug_morton(nb_ctx* c, uint32_t* codes, uint32_t* perm) {
   NB_TRY
   if (!c) return nb_fail(NB_E_ARG, "nb_debug_morton: null context");

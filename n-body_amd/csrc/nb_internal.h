@@ -212,6 +212,7 @@ struct nb_ctx {
   int bpl = 2;     // i-bodies per lane
   int jsplit = 2;  // waves per i-block
   bool guard = false;
+  int walk_stack = 0;  // BH traversal stack region cap (0: the full kStackCap); a test hook
 
   nb_bh_state bh;
   // Morton reordering (nb_reorder): order[slot] = the caller's index of the

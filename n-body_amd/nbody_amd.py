@@ -208,6 +208,7 @@ _SIGS = {
     "nb_load_snapshot": (C.c_int, [_P, C.c_char_p]),
     "nb_debug_morton": (C.c_int, [_P, _P, _P]),
     "nb_debug_tree": (C.c_int, [_P, _P, C.c_int64, C.POINTER(C.c_int64), _P]),
+    "nb_debug_set_walk_stack": (C.c_int, [_P, C.c_int32]),
     "nb_bh_forces_noalloc": (C.c_int, [_P, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_int32,
                                        C.c_int32, _P, _P, _P]),
     "nb_reorder": (C.c_int, [_P]),
@@ -540,6 +541,10 @@ class Simulation:
 
     def load_snapshot(self, path):
         _check(self._lib.nb_load_snapshot(self._ctx, str(path).encode()))
+
+    def debug_set_walk_stack(self, entries: int):
+        """Cap the traversal's LDS stack region (0 restores the full 512)."""
+        _check(self._lib.nb_debug_set_walk_stack(self._ctx, entries))
 
     def debug_morton(self):
         codes = np.empty(self.n, np.uint32)

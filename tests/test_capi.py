@@ -82,6 +82,7 @@ def test_argument_errors(nb):
     assert lib.nb_tune_leaf(None, 1, None) == nb.NB_E_ARG
     assert lib.nb_batch_set_quotas(None, None, None, 1) == nb.NB_E_ARG
     assert lib.nb_server_get_stats(None, None) == nb.NB_E_ARG
+    assert lib.nb_debug_set_walk_stack(None, 0) == nb.NB_E_ARG
 
 
 def test_initial_conditions_are_seeded(nb):

@@ -223,6 +223,27 @@ def test_forces_independent_of_traversal_order(nb):
     assert np.array_equal(f0, f1) and np.array_equal(f0, f2)
 
 
+def test_walk_stack_overflow_is_reported(nb):
+    # the overflow path, forced from the host with a tiny stack region: every
+    # wave stops its walk and exits normally, and the pass reports
+    # NB_E_TREE_OVERFLOW (sticky until the next upload)
+    n = 20000
+    pos, vel, m = nb.ic_plummer(41, n, 50.0, 1e4)
+    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH, leaf_size=1) as sim:
+        sim.upload(pos, vel, m)
+        ref = sim.forces_f64()
+        with pytest.raises(nb.NBodyError) as err:
+            sim.debug_set_walk_stack(513)
+        assert err.value.code == nb.NB_E_ARG
+        sim.debug_set_walk_stack(4)
+        with pytest.raises(nb.NBodyError) as err:
+            sim.forces_f64()
+        assert err.value.code == nb.NB_E_TREE_OVERFLOW
+        sim.debug_set_walk_stack(0)
+        sim.upload(pos, vel, m)
+        assert np.array_equal(sim.forces_f64(), ref)
+
+
 def test_c3_discs_properties(nb, ora):
     n = 1 << 22
     pos, vel, m = nb.ic_discs(2024, n)
