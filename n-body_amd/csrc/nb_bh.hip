@@ -1530,7 +1530,8 @@ __global__ __launch_bounds__(64 * kListWaves, kCount ? 5 : kListBlocksPerCU) voi
     w = __builtin_amdgcn_readfirstlane(w);
     if (w >= nrec) break;
     const uint64_t t_start = wall_clock64();
-    const int4 rec = waves[order[w]];  // wave-uniform: scalar registers
+    const int ri = order[w];
+    const int4 rec = waves[(unsigned)ri < (unsigned)nrec ? ri : w];  // wave-uniform: scalar registers
     const bool pair = rec.w > 0;
     // shared walk: a single range is held 64 / P1 times, a pair 32 / P1 times
     const int P1 = copy_lanes(pair ? max(rec.y, rec.w) : rec.y);
@@ -1668,11 +1669,13 @@ __global__ __launch_bounds__(256) void k_rec_bucket(const int4* __restrict__ wav
   if (threadIdx.x < kCostBuckets && h[threadIdx.x] > 0) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
 
-// One wave: cursor[b] = records in buckets above b (costliest first).
+// One wave: cursor[b] = records in buckets above b (costliest first); the
+// counts are cleared for the next pass's k_rec_bucket.
 __global__ __launch_bounds__(64) void k_rec_offsets(int* __restrict__ hist) {
   const int lane = threadIdx.x;
   const int v = hist[kCostBuckets - 1 - lane];
   hist[kCostBuckets + kCostBuckets - 1 - lane] = wave_incl_scan(v) - v;
+  hist[kCostBuckets - 1 - lane] = 0;
 }
 
 __global__ __launch_bounds__(256) void k_rec_scatter(const int* __restrict__ wctl, int n,
@@ -1697,8 +1700,10 @@ __global__ __launch_bounds__(256) void k_rec_scatter(const int* __restrict__ wct
       base[threadIdx.x] = atomicAdd(&hist[kCostBuckets + threadIdx.x], h[threadIdx.x]);
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < kPer; ++k)
-      if (bk[k] >= 0) order[base[bk[k]] + pos[k]] = c0 + k * 256 + threadIdx.x;
+    for (int k = 0; k < kPer; ++k) {
+      const int at = bk[k] >= 0 ? base[bk[k]] + pos[k] : nrec;
+      if (at < nrec) order[at] = c0 + k * 256 + threadIdx.x;
+    }
     __syncthreads();
   }
 }
