@@ -147,6 +147,25 @@ def test_root_leaf_with_several_ranges(nb, ora, n, leaf):
     assert rel_err(f, ref).max() < 1e-5
 
 
+def test_unsoftened_with_coincident_bodies(nb, ora):
+    # eps2 = 0 selects the guarded traversal: a pair at distance zero adds
+    # nothing (the CPU twin skips it), every other pair is unsoftened
+    n = 20000
+    pos, vel, m = nb.ic_plummer(51, n, 50.0, 1e4)
+    pos[100:110] = pos[99]  # eleven bodies at one point
+    p4 = pm4(pos, m)
+    with nb.Simulation(n, 0.0, DT, solver=nb.SOLVER_BH, leaf_size=4) as sim:
+        sim.upload(pos, vel, m)
+        f = sim.forces()
+        inter = sim.count_interactions()
+    assert np.all(np.isfinite(f))
+    mn, mx, ccodes, cperm, tree, com = cpu_build(ora, p4, 4)
+    fc, cinter = ora.bh_forces_group(p4[cperm], cperm, mn, mx, tree, com, 0.0,
+                                     group=nb.BH_WAVE_BODIES, codes_sorted=ccodes[cperm])
+    assert inter == cinter
+    assert np.percentile(rel_err(f, fc), 99) < 1e-5
+
+
 def test_deep_leaf_of_many_ranges(nb, ora):
     # 9000 coincident bodies make one level-10 leaf cut into 282 ranges (281 of
     # 32 bodies and one of 8): far more ranges of one width than a node's
