@@ -1283,16 +1283,6 @@ __device__ __forceinline__ float box_dist2(const Box& b, const float4& c) {
   return fmaf(ez, ez, fmaf(ey, ey, ex * ex));
 }
 
-// Inclusive prefix inside each row of 16 lanes (the first four steps of
-// wave_incl_scan).
-__device__ __forceinline__ int row_incl_scan(int v) {
-  v += dpp_shift<0x111, 0xf>(v);
-  v += dpp_shift<0x112, 0xf>(v);
-  v += dpp_shift<0x114, 0xf>(v);
-  v += dpp_shift<0x118, 0xf>(v);
-  return v;
-}
-
 // Walk the tree for the bodies held by the wave's lanes and accumulate their
 // accelerations (lanes of slice s take staged entries s, s + S, ...).
 //   kShared: the shared walk of a pair of ranges with boxes A and B. A cell
@@ -1343,13 +1333,13 @@ __device__ void walk(const WaveLds& L, int lane, int slice, int log2S, float4 me
     float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
     float rr = 0.f;
     if (!kShared && sp < 8 && dpos < nd) {
-      // deferred entries: up to 16 of them, at most 64 cells, listed in the
+      // deferred entries: up to 32 of them, at most 64 cells, listed in the
       // staging area (free while walking) so that lane j takes the j-th cell
       int2 d = make_int2(0, 0);
-      const bool have = lane < 16 && dpos + lane < nd;
+      const bool have = lane < 32 && dpos + lane < nd;
       if (have) d = stack[L.cap - 1 - (dpos + lane)];
       const int pc = __popc((uint32_t)d.y);
-      const int incl = row_incl_scan(pc);
+      const int incl = wave_incl_scan(pc);
       const uint64_t fit = __ballot(have && incl <= 64);
       const int kd = __popcll(fit);  // >= 1: one entry has at most 8 cells
       int* cells = reinterpret_cast<int*>(L.stage);
