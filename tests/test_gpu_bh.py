@@ -126,13 +126,19 @@ def test_small_and_ragged(nb, ora, n):
     pos, vel, m = nb.ic_ball(n, n, 10.0, 1.0)
     f, codes, perm, nodes, bbox, inter = gpu_run(nb, pos, vel, m, 4)
     p4 = pm4(pos, m)
-    _, _, ccodes, cperm, tree, _ = cpu_build(ora, p4, 4)
+    mn, mx, ccodes, cperm, tree, com = cpu_build(ora, p4, 4)
     np.testing.assert_array_equal(codes, ccodes)
     np.testing.assert_array_equal(perm, cperm)
     np.testing.assert_array_equal(nodes["next"], tree["next"])
     assert np.all(np.isfinite(f))
     if n == 1:
         assert np.all(f == 0)
+        return
+    # the pair walk against the CPU twin, which walks range by range
+    fc, cinter = ora.bh_forces_group(p4[cperm], cperm, mn, mx, tree, com, EPS2,
+                                     group=nb.BH_WAVE_BODIES, codes_sorted=ccodes[cperm])
+    assert inter == cinter
+    assert np.percentile(rel_err(f, fc), 99) < 1e-5
 
 
 @pytest.mark.parametrize("n,leaf", [(50, 64), (64, 64), (33, 40)])
