@@ -727,7 +727,8 @@ __device__ __forceinline__ int copy_lanes(int count) {
 
 // Number of records emit_records makes of one node's wave ranges.
 __device__ __forceinline__ int count_records(const int* starts, const int* ends, int nc, int l,
-                                             int leaf) {
+                                             int leaf, bool pairs) {
+  if (!pairs) return pack_ranges(starts, ends, nc, l, leaf, [](int, int, int) {});
   int per_width[4] = {0, 0, 0, 0};  // ranges of width 4, 8, 16, 32
   pack_ranges(starts, ends, nc, l, leaf, [&](int, int, int c) {
     const int k = 29 - __clz(copy_lanes(c));
@@ -743,10 +744,10 @@ __device__ __forceinline__ int count_records(const int* starts, const int* ends,
 // lanes), (start, count, start, count), or of one range, (start, count, 0, 0):
 // each range waits for the next of its width, in order. Writes count_records()
 // records from record `first` on. There are at most as many records as ranges,
-// and at most n ranges.
+// and at most n ranges. Without `pairs` every range is a record of its own.
 __device__ __forceinline__ void emit_records(int4* __restrict__ waves, int first, int n,
                                              const int* starts, const int* ends, int nc, int l,
-                                             int leaf) {
+                                             int leaf, bool pairs) {
   int2 wait[4];  // one waiting range per width 4, 8, 16, 32
 #pragma unroll
   for (int k = 0; k < 4; ++k) wait[k] = make_int2(0, 0);
@@ -755,6 +756,10 @@ __device__ __forceinline__ void emit_records(int4* __restrict__ waves, int first
     if (first + r < n) waves[first + r] = v;
     ++r;
   };
+  if (!pairs) {
+    pack_ranges(starts, ends, nc, l, leaf, [&](int, int a, int c) { put(make_int4(a, c, 0, 0)); });
+    return;
+  }
   pack_ranges(starts, ends, nc, l, leaf, [&](int, int a, int c) {
     const int k = 29 - __clz(copy_lanes(c));  // 4 -> 0, ..., 32 -> 3
 #pragma unroll
@@ -781,7 +786,7 @@ __global__ __launch_bounds__(256) void k_tree_emit(const uint32_t* __restrict__ 
                                                    const int* __restrict__ child_bounds,
                                                    int64_t max_nodes, int* __restrict__ flags,
                                                    int4* __restrict__ waves,
-                                                   int* __restrict__ wctl) {
+                                                   int* __restrict__ wctl, int pairs) {
   const int lo = level_off[l];
   const int hi = min(level_off[l + 1], lo + cap);
   const int next_lo = level_off[l + 1];
@@ -827,13 +832,13 @@ __global__ __launch_bounds__(256) void k_tree_emit(const uint32_t* __restrict__ 
     // wave ranges of the children of nodes holding more than kWaveBodies bodies,
     // in records of two: one atomic per wave reserves the records of all its lanes
     const bool packs = internal && link_count(nd.w) > kWaveBodies;
-    const int nrec = packs ? count_records(starts, ends, nc, l, leaf) : 0;
+    const int nrec = packs ? count_records(starts, ends, nc, l, leaf, pairs) : 0;
     const int incl = wave_incl_scan(nrec);
     const int total = __shfl(incl, 63);
     int wbase = 0;
     if (lane == 63 && total > 0) wbase = atomicAdd(&wctl[0], total);
     wbase = __shfl(wbase, 63);
-    if (nrec > 0) emit_records(waves, wbase + incl - nrec, n, starts, ends, nc, l, leaf);
+    if (nrec > 0) emit_records(waves, wbase + incl - nrec, n, starts, ends, nc, l, leaf, pairs);
   }
 }
 
@@ -883,7 +888,7 @@ __global__ __launch_bounds__(1024) void k_top_build(const int* __restrict__ top_
                                                     int* __restrict__ level_off,
                                                     int4* __restrict__ waves,
                                                     int* __restrict__ wctl,
-                                                    int* __restrict__ rec_hist) {
+                                                    int* __restrict__ rec_hist, int pairs) {
   __shared__ int s_idx[kTopCells];   // exclusive scan of existence, then node index
   __shared__ int s_next[kTopCells];  // rope of each node
   __shared__ int s_part[1024];
@@ -945,7 +950,7 @@ __global__ __launch_bounds__(1024) void k_top_build(const int* __restrict__ top_
       int r = 0;
       for (int o = 0; o < n; o += kWaveBodies) {
         const int k = min(kWaveBodies, n - o), o2 = o + kWaveBodies;
-        if (k == kWaveBodies && n - o2 >= kWaveBodies) {
+        if (pairs && k == kWaveBodies && n - o2 >= kWaveBodies) {
           waves[r++] = make_int4(o, k, o2, kWaveBodies);
           o = o2;
         } else {
@@ -992,8 +997,8 @@ __global__ __launch_bounds__(1024) void k_top_build(const int* __restrict__ top_
             ends[nc] = top_start[eb + d + 1];
             ++nc;
           }
-        const int nrec = count_records(starts, ends, nc, l, leaf);
-        emit_records(waves, atomicAdd(&s_ranges, nrec), n, starts, ends, nc, l, leaf);
+        const int nrec = count_records(starts, ends, nc, l, leaf, pairs);
+        emit_records(waves, atomicAdd(&s_ranges, nrec), n, starts, ends, nc, l, leaf, pairs);
       }
     }
     __syncthreads();
@@ -1874,8 +1879,9 @@ hipError_t nb_bh_build(nb_ctx* c) {
   const int leaf = c->cfg.leaf_size;
   hipLaunchKernelGGL(k_top_bounds, dim3((kTopEntries + 255) / 256), dim3(256), 0, s, codes_sorted,
                      n, b.top_start);
+  const int pairs = (c->cfg.flags & NB_F_BH_SINGLE_RANGES) ? 0 : 1;
   hipLaunchKernelGGL(k_top_build, dim3(1), dim3(1024), 0, s, b.top_start, n, leaf, b.node_link,
-                     b.level_off, b.waves, b.counters, b.rec_hist);
+                     b.level_off, b.waves, b.counters, b.rec_hist, pairs);
   for (int l = kTopLevels; l < kMaxLevel; ++l) {
     const int bound = level_bound(n, l);
     const int grid = std::max(1, std::min((bound + 255) / 256, 4096));
@@ -1886,7 +1892,7 @@ hipError_t nb_bh_build(nb_ctx* c) {
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_tree_emit, dim3(grid), dim3(256), 0, s, codes_sorted, n, b.node_link,
                        b.level_off, l, leaf, bound, b.child_off, b.child_bounds, b.max_nodes,
-                       c->flags, b.waves, b.counters);
+                       c->flags, b.waves, b.counters, pairs);
   }
   // 7. centre of mass
   const int cgrid = (int)std::max<int64_t>(1, std::min<int64_t>((b.max_nodes + 255) / 256, 8192));

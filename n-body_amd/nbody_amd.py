@@ -35,6 +35,7 @@ INTEG_EULER = 1
 F_NO_GRAPH = 0x1
 F_BH_PER_BODY = 0x4
 F_BH_AXIS_BOX = 0x8
+F_BH_SINGLE_RANGES = 0x10
 BH_WAVE_BODIES = 32  # NB_BH_WAVE_BODIES
 NOALLOC_MAX_BODIES = 65536  # NB_NOALLOC_MAX_BODIES
 BATCH_UNLIMITED = 0x7FFFFFFF  # NB_BATCH_UNLIMITED
@@ -179,7 +180,7 @@ class Tuning(C.Structure):
         ("bodies_per_lane", C.c_int32),
         ("j_split", C.c_int32),
         ("leaf_size", C.c_int32),
-        ("pad_", C.c_int32),
+        ("bh_single_ranges", C.c_int32),
         ("direct_ms", C.c_double),
         ("bh_ms", C.c_double),
     ]

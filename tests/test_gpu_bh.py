@@ -172,6 +172,23 @@ def test_unsoftened_with_coincident_bodies(nb, ora):
     assert np.percentile(rel_err(f, fc), 99) < 1e-5
 
 
+@pytest.mark.parametrize("ic", ["ball", "discs"])
+def test_single_ranges_equal_pairs(nb, ora, ic):
+    # NB_F_BH_SINGLE_RANGES walks every range alone: the same interactions as
+    # the pair walk, both equal to the range-by-range CPU twin
+    n = 65537 if ic == "ball" else 65536  # the two discs take an even count
+    pos, vel, m = nb.ic_ball(61, n, 100.0, 1.0) if ic == "ball" else nb.ic_discs(62, n)
+    p4 = pm4(pos, m)
+    fp, *_, ip = gpu_run(nb, pos, vel, m, 4)
+    fs, *_, is_ = gpu_run(nb, pos, vel, m, 4, flags=nb.F_BH_SINGLE_RANGES)
+    mn, mx, ccodes, cperm, tree, com = cpu_build(ora, p4, 4)
+    fc, cinter = ora.bh_forces_group(p4[cperm], cperm, mn, mx, tree, com, EPS2,
+                                     group=nb.BH_WAVE_BODIES, codes_sorted=ccodes[cperm])
+    assert ip == is_ == cinter
+    assert np.percentile(rel_err(fs, fc), 99) < 1e-5
+    assert np.percentile(rel_err(fp, fs), 99) < 1e-5
+
+
 def test_deep_leaf_of_many_ranges(nb, ora):
     # 9000 coincident bodies make one level-10 leaf cut into 282 ranges (281 of
     # 32 bodies and one of 8): far more ranges of one width than a node's
@@ -409,9 +426,12 @@ def test_per_body_count_and_leaf_tuning(nb, ora):
         assert performed >= per_body
         t = sim.tune_leaf(2)
         assert t["leaf_size"] in (1, 2, 4, 8, 16, 32) and t["bh_ms"] > 0
+        assert t["bh_single_ranges"] in (0, 1)
         f_tuned = sim.forces_f64()
         sim.step(2)
         sim.sync()
-    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH, leaf_size=t["leaf_size"]) as ref:
+    flags = nb.F_BH_SINGLE_RANGES if t["bh_single_ranges"] else 0
+    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH, leaf_size=t["leaf_size"],
+                       flags=flags) as ref:
         ref.upload(pos, vel, m)
         np.testing.assert_array_equal(ref.forces_f64(), f_tuned)
