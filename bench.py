@@ -193,7 +193,9 @@ def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0, accuracy: boo
     with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH) as sim:
         sim.upload(pos, vel, m)
         tuned = sim.tune_leaf(3)  # on these discs, not on a stand-in system
-        log(f"bh: leaf size {tuned['leaf_size']} tuned on the discs ({tuned['bh_ms']:.2f} ms force pass)")
+        walk = "single ranges" if tuned["bh_single_ranges"] else "pairs of ranges"
+        log(f"bh: leaf size {tuned['leaf_size']} with {walk} tuned on the discs "
+            f"({tuned['bh_ms']:.2f} ms force pass)")
         inter = sim.count_interactions()
         inter_pb = sim.count_interactions_per_body()
         log(f"bh: {inter:.4e} interactions performed, {inter_pb:.4e} needed body by body "
@@ -239,6 +241,7 @@ def bench_bh(nb, steps: int, warmup: int, cpu_budget: float = 0.0, accuracy: boo
         "workload": "Barnes-Hut N=4194304, two exponential discs (Rd=30, h=2, 300 apart, "
                     "closing 0.5/step), theta=0.5, eps2=40, dt=0.0035",
         "leaf_size": tuned["leaf_size"],
+        "single_ranges": bool(tuned["bh_single_ranges"]),
         "steps": steps,
         "ms_per_step": ms,
         "ms_per_step_after_reorder": reordered_ms,
