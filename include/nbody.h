@@ -101,13 +101,18 @@ typedef struct nb_config {
  * reference's morton.glsl:68-75 (cells become boxes with the root's aspect
  * ratio), instead of the default cubic root cell (extent = the largest axis). */
 #define NB_F_BH_AXIS_BOX 0x8
-/* Barnes-Hut: walk every wave range on its own. By default the build pairs
- * sibling ranges of equal copy width and a pair walks the tree once for both
- * (exactly the same interactions per range, summed in another order). That is
- * usually faster, but not always: on a uniform 4M-body ball at leaf size 4 it
- * was slower. nb_tune_leaf times both with every leaf size on the caller's
- * bodies and sets or clears this flag. */
+/* Barnes-Hut range pairing. The build can pair sibling wave ranges of equal
+ * copy width so that a pair walks the tree once for both (exactly the same
+ * interactions per range, summed in another order). By default it pairs, and
+ * orders the traversal's work costliest first, from NB_BH_PAIR_MIN_BODIES
+ * bodies on; below that there are too few ranges per wave for either to pay.
+ * NB_F_BH_PAIRS pairs at any size, NB_F_BH_SINGLE_RANGES never (it wins if
+ * both are set). Pairing is usually faster on large systems, but not always:
+ * on a uniform 4M-body ball at leaf size 4 it was slower. nb_tune_leaf times
+ * every leaf size both ways on the caller's bodies and sets one of the two. */
 #define NB_F_BH_SINGLE_RANGES 0x10
+#define NB_F_BH_PAIRS 0x20
+#define NB_BH_PAIR_MIN_BODIES (1 << 20)
 
 typedef struct nb_ctx nb_ctx;
 
@@ -408,7 +413,8 @@ typedef struct nb_tuning {
   int32_t bodies_per_lane;
   int32_t j_split;
   int32_t leaf_size;
-  int32_t bh_single_ranges;  /* nb_tune_leaf: 1 if single ranges timed faster than pairs */
+  int32_t bh_single_ranges;  /* nb_tune_leaf: 1 if single ranges timed faster than pairs
+                                (NB_F_BH_SINGLE_RANGES set), 0 if pairs (NB_F_BH_PAIRS set) */
   double direct_ms;  /* force pass at the chosen shape (0 if not tuned) */
   double bh_ms;      /* Barnes-Hut build + traversal at the chosen leaf size */
 } nb_tuning;

@@ -1525,7 +1525,7 @@ __global__ __launch_bounds__(64 * kListWaves, kCount ? 5 : kListBlocksPerCU) voi
     w = __builtin_amdgcn_readfirstlane(w);
     if (w >= nrec) break;
     const uint64_t t_start = wall_clock64();
-    const int ri = order[w];
+    const int ri = order ? order[w] : w;
     const int4 rec = waves[(unsigned)ri < (unsigned)nrec ? ri : w];  // wave-uniform: scalar registers
     const bool pair = rec.w > 0;
     // shared walk: a single range is held 64 / P1 times, a pair 32 / P1 times
@@ -1603,7 +1603,7 @@ __global__ __launch_bounds__(64 * kListWaves, kCount ? 5 : kListBlocksPerCU) voi
     }
     // the record's device time, shared out over its bodies for the next pass's order
     const float share = (float)(wall_clock64() - t_start) / (float)(rec.y + rec.w);
-    if (valid && lane < grp) {
+    if (cost && valid && lane < grp) {
       const uint32_t o = perm[t];
       if (o < (uint32_t)n) cost[o] = share;
     }
@@ -1879,7 +1879,7 @@ hipError_t nb_bh_build(nb_ctx* c) {
   const int leaf = c->cfg.leaf_size;
   hipLaunchKernelGGL(k_top_bounds, dim3((kTopEntries + 255) / 256), dim3(256), 0, s, codes_sorted,
                      n, b.top_start);
-  const int pairs = (c->cfg.flags & NB_F_BH_SINGLE_RANGES) ? 0 : 1;
+  const int pairs = nb_bh_pairs(n, c->cfg.flags) ? 1 : 0;
   hipLaunchKernelGGL(k_top_build, dim3(1), dim3(1024), 0, s, b.top_start, n, leaf, b.node_link,
                      b.level_off, b.waves, b.counters, b.rec_hist, pairs);
   for (int l = kTopLevels; l < kMaxLevel; ++l) {
@@ -1915,7 +1915,10 @@ hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions) {
       c->cfg.eps2, c->G, c->force, b.inter, c->flags
   const bool guard = c->guard;
   const int gl = b.persist_blocks;  // persistent blocks pulling wave ranges
-  if (group) {
+  // work costliest first where there are many records per wave (the sizes at
+  // which ranges pair); otherwise in record order, and no times are kept
+  const bool ordered = group && nb_bh_pairs(n, c->cfg.flags);
+  if (ordered) {
     const int og = std::max(1, std::min((n + kOrderChunk - 1) / kOrderChunk, kOrderBlocks));
     hipLaunchKernelGGL(k_rec_bucket, dim3(og), dim3(256), 0, c->stream, b.waves, b.counters, n,
                        b.rec_cost, b.vals[0], b.rec_bucket, b.rec_hist);
@@ -1925,8 +1928,8 @@ hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions) {
   }
 #define NB_LIST_ARGS                                                                     \
   b.pms, b.vals[0], n, b.node_com, b.node_link, b.level_off, b.max_nodes, b.node_r2,     \
-      c->cfg.eps2, c->G, c->force, b.inter, c->flags, b.waves, b.counters, b.rec_order,  \
-      b.rec_cost, c->walk_stack
+      c->cfg.eps2, c->G, c->force, b.inter, c->flags, b.waves, b.counters,                \
+      ordered ? b.rec_order : nullptr, ordered ? b.rec_cost : nullptr, c->walk_stack
   if (group && count_interactions && guard)
     hipLaunchKernelGGL((k_traverse_list<true, true>), dim3(gl), dim3(64 * kListWaves), 0, c->stream,
                        NB_LIST_ARGS);

@@ -83,6 +83,12 @@ constexpr int kTopLevels = 4;                         // levels built from per-c
 constexpr int kTopCells = 1 + 8 + 64 + 512 + 4096;    // cells of levels 0..kTopLevels
 constexpr int kTopEntries = kTopCells + kTopLevels + 1;  // plus an end sentinel per level
 constexpr int kCostBuckets = 64;  // half-octaves of estimated record time
+// Whether the Barnes-Hut build pairs sibling ranges (and the traversal orders
+// its work): by size unless a flag forces it either way.
+inline bool nb_bh_pairs(int64_t n, int flags) {
+  if (flags & NB_F_BH_SINGLE_RANGES) return false;
+  return (flags & NB_F_BH_PAIRS) || n >= NB_BH_PAIR_MIN_BODIES;
+}
 
 enum nb_bh_buf {
   kBhBbox, kBhGeom, kBhKey0, kBhKey1, kBhVal0, kBhVal1, kBhCodes, kBhHist, kBhIscan,

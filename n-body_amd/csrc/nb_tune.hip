@@ -165,13 +165,12 @@ extern "C" int nb_tune_leaf(nb_ctx* c, int32_t reps, nb_tuning* out) {
   // Candidates are timed in interleaved rounds (one pass of each per round, the
   // first round untimed) and each keeps its fastest pass, so clock and
   // neighbour drift during the sweep affects every candidate alike.
-  // Each leaf size is timed with sibling ranges walked in pairs and alone
-  // (NB_F_BH_SINGLE_RANGES): pairs win on clustered systems, single ranges on
-  // near-uniform ones.
+  // Each leaf size is timed with sibling ranges walked in pairs
+  // (NB_F_BH_PAIRS) and alone (NB_F_BH_SINGLE_RANGES).
   constexpr int kLeaves[] = {1, 2, 4, 8, 16, 32};
   constexpr int kLeafCount = sizeof(kLeaves) / sizeof(kLeaves[0]);
   constexpr int kCount = 2 * kLeafCount;
-  const int flags0 = c->cfg.flags & ~NB_F_BH_SINGLE_RANGES;
+  const int flags0 = c->cfg.flags & ~(NB_F_BH_SINGLE_RANGES | NB_F_BH_PAIRS);
   double ms[kCount];
   for (double& m : ms) m = DBL_MAX;
   int rc = NB_OK, best_leaf = c->cfg.leaf_size, best_single = 0;
@@ -179,7 +178,7 @@ extern "C" int nb_tune_leaf(nb_ctx* c, int32_t reps, nb_tuning* out) {
   for (int r = -1; !rc && r < reps; ++r) {
     for (int k = 0; !rc && k < kCount; ++k) {
       c->cfg.leaf_size = kLeaves[k % kLeafCount];
-      c->cfg.flags = flags0 | (k >= kLeafCount ? NB_F_BH_SINGLE_RANGES : 0);
+      c->cfg.flags = flags0 | (k >= kLeafCount ? NB_F_BH_SINGLE_RANGES : NB_F_BH_PAIRS);
       (void)hipEventRecord(e0, c->stream);
       hipError_t e = nb_bh_build(c);
       if (e == hipSuccess) e = nb_bh_traverse(c, false);
@@ -200,7 +199,7 @@ extern "C" int nb_tune_leaf(nb_ctx* c, int32_t reps, nb_tuning* out) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c->cfg.leaf_size = best_leaf;
-  c->cfg.flags = flags0 | (best_single ? NB_F_BH_SINGLE_RANGES : 0);
+  c->cfg.flags = flags0 | (best_single ? NB_F_BH_SINGLE_RANGES : NB_F_BH_PAIRS);
   // the captured step graph holds the old leaf size; the tree and forces of the
   // last timed pass are the last candidate's, so the chosen one runs once more
   nb_drop_graph(c->stream, c->graph_exec, c->graph);

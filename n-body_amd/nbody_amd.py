@@ -36,6 +36,8 @@ F_NO_GRAPH = 0x1
 F_BH_PER_BODY = 0x4
 F_BH_AXIS_BOX = 0x8
 F_BH_SINGLE_RANGES = 0x10
+F_BH_PAIRS = 0x20
+BH_PAIR_MIN_BODIES = 1 << 20  # NB_BH_PAIR_MIN_BODIES
 BH_WAVE_BODIES = 32  # NB_BH_WAVE_BODIES
 NOALLOC_MAX_BODIES = 65536  # NB_NOALLOC_MAX_BODIES
 BATCH_UNLIMITED = 0x7FFFFFFF  # NB_BATCH_UNLIMITED

@@ -122,9 +122,12 @@ def test_theta_tiny_is_direct_sum(nb, ora):
 # 8193 and 65537 put the level scans and then the radix-histogram scans just
 # past one workgroup's 8192 elements (two launches with a nearly empty tile)
 @pytest.mark.parametrize("n", [1, 2, 5, 4097, 8193, 65537])
-def test_small_and_ragged(nb, ora, n):
+@pytest.mark.parametrize("pairs", [False, True])
+def test_small_and_ragged(nb, ora, n, pairs):
+    # below NB_BH_PAIR_MIN_BODIES ranges walk alone unless NB_F_BH_PAIRS
     pos, vel, m = nb.ic_ball(n, n, 10.0, 1.0)
-    f, codes, perm, nodes, bbox, inter = gpu_run(nb, pos, vel, m, 4)
+    f, codes, perm, nodes, bbox, inter = gpu_run(nb, pos, vel, m, 4,
+                                                 flags=nb.F_BH_PAIRS if pairs else 0)
     p4 = pm4(pos, m)
     mn, mx, ccodes, cperm, tree, com = cpu_build(ora, p4, 4)
     np.testing.assert_array_equal(codes, ccodes)
@@ -141,26 +144,30 @@ def test_small_and_ragged(nb, ora, n):
     assert np.percentile(rel_err(f, fc), 99) < 1e-5
 
 
+@pytest.mark.parametrize("flags", [0, 0x20])
 @pytest.mark.parametrize("n,leaf", [(50, 64), (64, 64), (33, 40)])
-def test_root_leaf_with_several_ranges(nb, ora, n, leaf):
+def test_root_leaf_with_several_ranges(nb, ora, n, leaf, flags):
     # more bodies than one range holds but no more than one leaf: the root is a
-    # leaf cut into chunks, and every pair is evaluated
+    # leaf cut into chunks (two full chunks make a pair under NB_F_BH_PAIRS =
+    # 0x20), and every pair of bodies is evaluated
     pos, vel, m = nb.ic_ball(70 + n, n, 10.0, 1.0)
-    f, codes, perm, nodes, bbox, inter = gpu_run(nb, pos, vel, m, leaf)
+    f, codes, perm, nodes, bbox, inter = gpu_run(nb, pos, vel, m, leaf, flags=flags)
     assert len(nodes) == 1 and nodes["first_child"][0] == -1
     assert inter == n * (n - 1)
     ref = ora.direct_f64(pos.astype(np.float32).astype(np.float64), m.astype(np.float64), EPS2)
     assert rel_err(f, ref).max() < 1e-5
 
 
-def test_unsoftened_with_coincident_bodies(nb, ora):
+@pytest.mark.parametrize("flags", [0, 0x20])
+def test_unsoftened_with_coincident_bodies(nb, ora, flags):
     # eps2 = 0 selects the guarded traversal: a pair at distance zero adds
-    # nothing (the CPU twin skips it), every other pair is unsoftened
+    # nothing (the CPU twin skips it), every other pair is unsoftened; single
+    # ranges and (NB_F_BH_PAIRS = 0x20) pairs
     n = 20000
     pos, vel, m = nb.ic_plummer(51, n, 50.0, 1e4)
     pos[100:110] = pos[99]  # eleven bodies at one point
     p4 = pm4(pos, m)
-    with nb.Simulation(n, 0.0, DT, solver=nb.SOLVER_BH, leaf_size=4) as sim:
+    with nb.Simulation(n, 0.0, DT, solver=nb.SOLVER_BH, leaf_size=4, flags=flags) as sim:
         sim.upload(pos, vel, m)
         f = sim.forces()
         inter = sim.count_interactions()
@@ -179,7 +186,7 @@ def test_single_ranges_equal_pairs(nb, ora, ic):
     n = 65537 if ic == "ball" else 65536  # the two discs take an even count
     pos, vel, m = nb.ic_ball(61, n, 100.0, 1.0) if ic == "ball" else nb.ic_discs(62, n)
     p4 = pm4(pos, m)
-    fp, *_, ip = gpu_run(nb, pos, vel, m, 4)
+    fp, *_, ip = gpu_run(nb, pos, vel, m, 4, flags=nb.F_BH_PAIRS)
     fs, *_, is_ = gpu_run(nb, pos, vel, m, 4, flags=nb.F_BH_SINGLE_RANGES)
     mn, mx, ccodes, cperm, tree, com = cpu_build(ora, p4, 4)
     fc, cinter = ora.bh_forces_group(p4[cperm], cperm, mn, mx, tree, com, EPS2,
@@ -198,7 +205,8 @@ def test_deep_leaf_of_many_ranges(nb, ora):
     pos = np.concatenate([np.tile(np.array([[3.0, -2.0, 1.0]], np.float32), (n_pt, 1)), bp])
     m = np.concatenate([np.full(n_pt, 0.5, np.float32), bm])
     n = len(m)
-    f, codes, perm, nodes, bbox, inter = gpu_run(nb, pos, np.zeros((n, 3)), m, 4)
+    f, codes, perm, nodes, bbox, inter = gpu_run(nb, pos, np.zeros((n, 3)), m, 4,
+                                                 flags=nb.F_BH_PAIRS)
     assert (nodes["body_count"][nodes["level"] == 10] >= n_pt).any()
     p4 = pm4(pos, m)
     mn, mx, ccodes, cperm, tree, com = cpu_build(ora, p4, 4)
@@ -255,7 +263,9 @@ def test_forces_independent_of_traversal_order(nb):
     # bit for bit
     n = 200000
     pos, vel, m = nb.ic_discs(9, n)
-    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH, leaf_size=1) as sim:
+    # NB_F_BH_PAIRS: pairs, and with them the costliest-first order, below
+    # NB_BH_PAIR_MIN_BODIES too
+    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH, leaf_size=1, flags=nb.F_BH_PAIRS) as sim:
         sim.upload(pos, vel, m)
         f0 = sim.forces_f64()
         f1 = sim.forces_f64()
@@ -265,13 +275,15 @@ def test_forces_independent_of_traversal_order(nb):
     assert np.array_equal(f0, f1) and np.array_equal(f0, f2)
 
 
-def test_walk_stack_overflow_is_reported(nb):
+@pytest.mark.parametrize("flags", [0, 0x20])
+def test_walk_stack_overflow_is_reported(nb, flags):
     # the overflow path, forced from the host with a tiny stack region: every
     # wave stops its walk and exits normally, and the pass reports
-    # NB_E_TREE_OVERFLOW (sticky until the next upload)
+    # NB_E_TREE_OVERFLOW (sticky until the next upload); single ranges and
+    # (NB_F_BH_PAIRS = 0x20) pairs, whose deferred entries share the region
     n = 20000
     pos, vel, m = nb.ic_plummer(41, n, 50.0, 1e4)
-    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH, leaf_size=1) as sim:
+    with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH, leaf_size=1, flags=flags) as sim:
         sim.upload(pos, vel, m)
         ref = sim.forces_f64()
         with pytest.raises(nb.NBodyError) as err:
@@ -430,7 +442,7 @@ def test_per_body_count_and_leaf_tuning(nb, ora):
         f_tuned = sim.forces_f64()
         sim.step(2)
         sim.sync()
-    flags = nb.F_BH_SINGLE_RANGES if t["bh_single_ranges"] else 0
+    flags = nb.F_BH_SINGLE_RANGES if t["bh_single_ranges"] else nb.F_BH_PAIRS
     with nb.Simulation(n, EPS2, DT, solver=nb.SOLVER_BH, leaf_size=t["leaf_size"],
                        flags=flags) as ref:
         ref.upload(pos, vel, m)
