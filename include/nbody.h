@@ -104,8 +104,9 @@ typedef struct nb_config {
 /* Barnes-Hut range pairing. The build can pair sibling wave ranges of equal
  * copy width so that a pair walks the tree once for both (exactly the same
  * interactions per range, summed in another order). By default it pairs, and
- * orders the traversal's work costliest first, from NB_BH_PAIR_MIN_BODIES
- * bodies on; below that there are too few ranges per wave for either to pay.
+ * the traversal takes its work costliest first (paired or not), from
+ * NB_BH_PAIR_MIN_BODIES bodies on; below that there are too few ranges per
+ * wave for either to pay.
  * NB_F_BH_PAIRS pairs at any size, NB_F_BH_SINGLE_RANGES never (it wins if
  * both are set). Pairing is usually faster on large systems, but not always:
  * on a uniform 4M-body ball at leaf size 4 it was slower. nb_tune_leaf times

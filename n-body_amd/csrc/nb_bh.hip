@@ -1916,8 +1916,10 @@ hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions) {
   const bool guard = c->guard;
   const int gl = b.persist_blocks;  // persistent blocks pulling wave ranges
   // work costliest first where there are many records per wave (the sizes at
-  // which ranges pair); otherwise in record order, and no times are kept
-  const bool ordered = group && nb_bh_pairs(n, c->cfg.flags);
+  // which ranges pair, whether they do or not, or wherever pairs are forced);
+  // otherwise in record order, and no times are kept
+  const bool ordered =
+      group && (n >= NB_BH_PAIR_MIN_BODIES || (c->cfg.flags & NB_F_BH_PAIRS) != 0);
   if (ordered) {
     const int og = std::max(1, std::min((n + kOrderChunk - 1) / kOrderChunk, kOrderBlocks));
     hipLaunchKernelGGL(k_rec_bucket, dim3(og), dim3(256), 0, c->stream, b.waves, b.counters, n,

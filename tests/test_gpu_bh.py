@@ -179,11 +179,11 @@ def test_unsoftened_with_coincident_bodies(nb, ora, flags):
     assert np.percentile(rel_err(f, fc), 99) < 1e-5
 
 
-@pytest.mark.parametrize("ic", ["ball", "discs"])
-def test_single_ranges_equal_pairs(nb, ora, ic):
+@pytest.mark.parametrize("ic,n", [("ball", 65537), ("discs", 65536), ("discs", 1 << 20)])
+def test_single_ranges_equal_pairs(nb, ora, ic, n):
     # NB_F_BH_SINGLE_RANGES walks every range alone: the same interactions as
-    # the pair walk, both equal to the range-by-range CPU twin
-    n = 65537 if ic == "ball" else 65536  # the two discs take an even count
+    # the pair walk, both equal to the range-by-range CPU twin (at 2^20 bodies
+    # both take their work costliest first)
     pos, vel, m = nb.ic_ball(61, n, 100.0, 1.0) if ic == "ball" else nb.ic_discs(62, n)
     p4 = pm4(pos, m)
     fp, *_, ip = gpu_run(nb, pos, vel, m, 4, flags=nb.F_BH_PAIRS)
