@@ -103,10 +103,9 @@ typedef struct nb_config {
 #define NB_F_BH_AXIS_BOX 0x8
 /* Barnes-Hut range pairing. The build can pair sibling wave ranges of equal
  * copy width so that a pair walks the tree once for both (exactly the same
- * interactions per range, summed in another order). By default it pairs, and
- * the traversal takes its work costliest first (paired or not), from
+ * interactions per range, summed in another order). By default it pairs from
  * NB_BH_PAIR_MIN_BODIES bodies on; below that there are too few ranges per
- * wave for either to pay.
+ * wave for a pair's two phases to pay.
  * NB_F_BH_PAIRS pairs at any size, NB_F_BH_SINGLE_RANGES never (it wins if
  * both are set). Pairing is usually faster on large systems, but not always:
  * on a uniform 4M-body ball at leaf size 4 it was slower. nb_tune_leaf times

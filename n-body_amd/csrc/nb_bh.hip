@@ -1641,6 +1641,11 @@ __device__ __forceinline__ int cost_bucket(float est) {
 // chunk, so the few busy buckets see few atomics.
 constexpr int kOrderChunk = 1024;
 constexpr int kOrderBlocks = 256;
+// Below this the three ordering launches cost more than the tail they save
+// (Plummer steps with single ranges, ordered against not: 0.577 against 0.583
+// ms at 65,536 bodies, 0.858 against 0.881 at 262,144, 1.267 against 1.296 at
+// 524,288).
+constexpr int kOrderMinBodies = 1 << 16;
 
 __global__ __launch_bounds__(256) void k_rec_bucket(const int4* __restrict__ waves,
                                                     const int* __restrict__ wctl, int n,
@@ -1915,11 +1920,9 @@ hipError_t nb_bh_traverse(nb_ctx* c, bool count_interactions) {
       c->cfg.eps2, c->G, c->force, b.inter, c->flags
   const bool guard = c->guard;
   const int gl = b.persist_blocks;  // persistent blocks pulling wave ranges
-  // work costliest first where there are many records per wave (the sizes at
-  // which ranges pair, whether they do or not, or wherever pairs are forced);
-  // otherwise in record order, and no times are kept
-  const bool ordered =
-      group && (n >= NB_BH_PAIR_MIN_BODIES || (c->cfg.flags & NB_F_BH_PAIRS) != 0);
+  // work costliest first from kOrderMinBodies on (and wherever pairs are
+  // forced); smaller systems take their records in order and keep no times
+  const bool ordered = group && (n >= kOrderMinBodies || (c->cfg.flags & NB_F_BH_PAIRS) != 0);
   if (ordered) {
     const int og = std::max(1, std::min((n + kOrderChunk - 1) / kOrderChunk, kOrderBlocks));
     hipLaunchKernelGGL(k_rec_bucket, dim3(og), dim3(256), 0, c->stream, b.waves, b.counters, n,
