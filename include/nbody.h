@@ -426,12 +426,13 @@ int nb_autotune(int64_t n, int32_t what, int32_t reps, nb_tuning* out);
  * particle state is not changed) and keep the fastest for later steps.
  * out->direct_ms = its force pass, out->n = bodies in the batch. */
 int nb_batch_tune(nb_batch* b, int32_t reps, nb_tuning* out);
-/* Barnes-Hut leaf size tuned on the workload itself: times leaf_size in
- * {1,2,4,8,16,32} on the context's current bodies (the particle state is not
- * changed): one untimed round, then reps rounds of one force pass per candidate,
- * interleaved, each candidate keeping its fastest pass. The fastest candidate
- * serves the context's later steps; out->leaf_size and out->bh_ms (its fastest
- * pass) receive the choice. */
+/* Barnes-Hut leaf size and range pairing tuned on the workload itself: times
+ * leaf_size in {1,2,4,8,16,32}, each with NB_F_BH_PAIRS and with
+ * NB_F_BH_SINGLE_RANGES, on the context's current bodies (the particle state
+ * is not changed): one untimed round, then reps rounds of one force pass per
+ * candidate, interleaved, each candidate keeping its fastest pass. The fastest
+ * candidate serves the context's later steps; out->leaf_size,
+ * out->bh_single_ranges and out->bh_ms (its fastest pass) receive the choice. */
 int nb_tune_leaf(nb_ctx* ctx, int32_t reps, nb_tuning* out);
 /* The defaults nb_create would use for n bodies (tuned or built in). */
 int nb_get_tuning(int64_t n, nb_tuning* out);
