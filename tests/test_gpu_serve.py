@@ -41,6 +41,22 @@ def test_alone_equals_mix(nb):
             assert np.array_equal(a, b)
 
 
+def test_large_bh_job_matches_standalone(nb):
+    # a job of 2^20 bodies walks its ranges in pairs (NB_BH_PAIR_MIN_BODIES)
+    # with its work ordered by the previous step's times; served beside a
+    # small job on a pooled context it equals a standalone context bit for bit
+    jobs = [(1 << 20, 3, 0, 11), (3000, 3, 1, 2)]
+    mix, st = run_server(nb, jobs)
+    assert st[0]["solver"] == nb.SOLVER_BH and st[0]["state"] == nb.JOB_DONE
+    n, steps, _, seed = jobs[0]
+    pos, vel, m = nb.ic_plummer(seed, n, 50.0, 1e4)
+    with nb.Simulation(n, 40.0, 0.0035, solver=nb.SOLVER_BH) as sim:
+        sim.upload(pos, vel, m)
+        sim.step(steps)
+        p, v, _ = sim.download_f64()
+    assert np.array_equal(p, mix[0][0]) and np.array_equal(v, mix[0][1])
+
+
 def test_jobs_match_standalone(nb, ora):
     mix, st = run_server(nb, JOBS[:5])
     for k, (n, steps, ic, seed) in enumerate(JOBS[:5]):
