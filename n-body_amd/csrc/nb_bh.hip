@@ -1177,11 +1177,12 @@ __global__ __launch_bounds__(256) void k_traverse(const float4* __restrict__ pms
 // opened children; no source data is read while walking. Whenever the list
 // fills up, and at the end, the wave gathers the sources 64 at a time into LDS
 // and streams them through the direct-sum inner loop (LDS broadcast reads, 13
-// VALU per interaction). The build pairs sibling ranges of equal copy width:
-// such a pair walks once for both ranges and leaves to each range only the
-// cells the two decide differently (walk() below), which on the 4M two-disc
-// system takes 30% fewer walk iterations and stages 34% fewer sources for
-// exactly the same interactions.
+// VALU per interaction). From NB_BH_PAIR_MIN_BODIES bodies on (or with
+// NB_F_BH_PAIRS) the build pairs sibling ranges of equal copy width: such a
+// pair walks once for both ranges and leaves to each range only the cells the
+// two decide differently (walk() below), which on the 4M two-disc system takes
+// 30% fewer walk iterations and stages 34% fewer sources for exactly the same
+// interactions.
 constexpr int kListWaves = 4;
 // LDS (26.7 KB per block) and 80 VGPRs allow six blocks per CU. A 384-entry
 // list (instead of 512) is what makes the sixth fit; it measured about 1% faster
