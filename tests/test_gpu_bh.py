@@ -359,6 +359,10 @@ def test_axis_box_matches_reference_quantisation(nb, ora):
     fc, cinter = ora.bh_forces_group(p4[cperm], cperm, mn, mx, tree, com, EPS2,
                                      group=nb.BH_WAVE_BODIES, codes_sorted=srt, cubic=False)
     assert np.percentile(rel_err(f, fc), 99) < 1e-5
+    assert inter == cinter
+    # the same boxes walked in pairs: the same interactions
+    fp, *_, ip = gpu_run(nb, pos, vel, m, 8, flags=nb.F_BH_AXIS_BOX | nb.F_BH_PAIRS)
+    assert ip == cinter and np.percentile(rel_err(fp, fc), 99) < 1e-5
     # the cubic default opens fewer cells on flat systems
     _, _, _, _, _, inter_cubic = gpu_run(nb, pos, vel, m, 8)
     assert inter_cubic < inter
